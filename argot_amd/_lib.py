@@ -1,0 +1,90 @@
+"""Loader for libargot_hip.so (built in-tree by `make -C argot_amd/csrc`).
+
+There is no fallback: if the library is missing, or a GPU entry point fails,
+the calls raise.
+"""
+import ctypes as C
+import os
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libargot_hip.so")
+
+# Entry points that include/argot_hip.h declares (checked by tests).
+EXPORTS = [
+    "setup_freq_param", "setup_photon_rate", "setup_cross_section",
+    "init_gpu", "send_mesh_data", "recv_mesh_data", "send_run_param_data",
+    "allocate_pinned_segment", "deallocate_pinned_segment",
+    "allocate_pinned_light_ray_IO", "deallocate_pinned_light_ray_IO",
+    "calc_optical_depth_cuda", "calc_photoion_rate_cuda", "zero_out_photoion_rate_cuda",
+    "merge_photoion_data", "step_radiation_tree",
+    "argot_hip_last_error", "argot_hip_error_string", "argot_hip_set_helium_fact",
+    "argot_hip_set_tau_mode", "argot_hip_set_rate_mode", "argot_hip_step_stats",
+]
+
+_lib = None
+
+
+class ArgotError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ArgotError(f"{LIB_PATH} is not built (run `make -C argot_amd/csrc`)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.setup_freq_param.argtypes = [P(abi.FreqParam)]
+    L.setup_photon_rate.argtypes = [P(abi.FreqParam), P(abi.RadiationSrc), C.c_double]
+    L.setup_cross_section.argtypes = [P(abi.CrossSection), P(abi.FreqParam)]
+    cm, cp, rp = P(abi.CudaMemSpace), P(abi.CudaParam), P(abi.RunParam)
+    L.init_gpu.argtypes = [C.c_void_p, cm, cp, rp]
+    L.send_mesh_data.argtypes = [C.c_void_p, cm, cp, rp]
+    L.recv_mesh_data.argtypes = [C.c_void_p, cm, cp, rp]
+    L.send_run_param_data.argtypes = [rp, cm, cp]
+    L.allocate_pinned_segment.argtypes = [P(C.c_void_p), C.c_uint64]
+    L.deallocate_pinned_segment.argtypes = [C.c_void_p]
+    L.allocate_pinned_light_ray_IO.argtypes = [P(C.c_void_p), C.c_uint64]
+    L.deallocate_pinned_light_ray_IO.argtypes = [C.c_void_p]
+    L.calc_optical_depth_cuda.argtypes = [C.c_void_p, cm, cp, rp]
+    L.calc_photoion_rate_cuda.argtypes = [cm, C.c_void_p, C.c_void_p, cp, rp]
+    L.zero_out_photoion_rate_cuda.argtypes = [cm, cp, rp]
+    L.merge_photoion_data.argtypes = [C.c_void_p, cm, cp, rp]
+    L.step_radiation_tree.argtypes = [C.c_void_p, C.c_void_p, rp, C.c_void_p, cm, cp, C.c_float]
+    L.argot_hip_last_error.restype = C.c_int
+    L.argot_hip_error_string.argtypes = [C.c_int]
+    L.argot_hip_error_string.restype = C.c_char_p
+    L.argot_hip_set_helium_fact.argtypes = [C.c_double]
+    L.argot_hip_set_tau_mode.argtypes = [C.c_int]
+    L.argot_hip_set_rate_mode.argtypes = [C.c_int]
+    L.argot_hip_step_stats.argtypes = [P(abi.StepStats)]
+    L.argot_hip_set_theta.argtypes = [C.c_float]
+    L.argot_hip_set_blocks_per_cu.argtypes = [C.c_int]
+    L.argot_hip_set_capture.argtypes = [C.c_void_p, C.c_int64]
+    L.argot_hip_record_size.restype = C.c_int64
+    L.argot_hip_abi_layout.argtypes = [P(C.c_int64), C.c_int]
+    L.argot_hip_abi_layout.restype = C.c_int
+    L.argot_hip_tree_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.c_float]
+    L.argot_hip_tree_build.restype = C.c_void_p
+    L.argot_hip_tree_free.argtypes = [C.c_void_p]
+    L.argot_hip_tree_nnode.argtypes = [C.c_void_p]
+    L.argot_hip_tree_rays_to.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+    L.argot_hip_tree_rays_to.restype = C.c_uint64
+    L.argot_hip_tree_spec.argtypes = [C.c_void_p, C.c_int]
+    L.argot_hip_tree_spec.restype = P(C.c_double)
+    _lib = L
+    return L
+
+
+def check(what):
+    """Raise if the last entry point reported an error."""
+    e = lib().argot_hip_last_error()
+    if e:
+        msg = lib().argot_hip_error_string(e).decode()
+        raise ArgotError(f"{what} failed: {msg} (code {e})")

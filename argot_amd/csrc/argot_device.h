@@ -1,0 +1,382 @@
+// Device-side building blocks of the point-source radiation step (gfx950).
+//
+// Everything here reproduces the reference's arithmetic operation by
+// operation: the file is compiled with -ffp-contract=off and without
+// fast-math, so a*b+c stays two roundings, float division and sqrt are
+// IEEE, and every double promotion of the C source is written out.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "argot_internal.h"
+
+namespace argot {
+
+constexpr double kWalkTiny = 1.0e-31;   // calc_optical_depth.c:15
+constexpr double kSegTiny = 1.0e-30;    // calc_ray_segment.c:12
+constexpr double kRateTiny = 1.0e-31;   // calc_photoion_rate.c:16
+constexpr double kPi = 3.1415926535897932384626433;  // constants.h:24
+constexpr double kHnuL = 6.626e-27 * 3.28e15;          // hplanck*nuL
+
+// Quotient a/b rounded to float exactly as IEEE float division does, from
+// the double reciprocal rb = 1.0/(double)b (rounded to double).  The
+// double product is within 2^-52 (relative) of a/b, while a quotient of two
+// 24-bit floats is never closer than 2^-49 (relative) to a rounding
+// boundary of float, so rounding the product to float gives RN(a/b).
+__device__ __forceinline__ float div_by(float a, double rb) {
+  return (float)((double)a * rb);
+}
+
+// Target cell centre, setup_light_ray_tree.c:341-343 / calc_ray_segment.c:48:
+// (float)i + 0.5 is double, the sum is rounded to float once.
+__device__ __forceinline__ float cell_centre(float lo, int i, float h) {
+  return (float)((double)lo + ((double)(float)i + 0.5) * (double)h);
+}
+
+struct Vec3f {
+  float x, y, z;
+};
+
+// calc_ray_segment.c:15-213 for one rank: start point of the segment whose
+// optical depth the ray keeps (a source outside the box enters at a face;
+// the domain loop can split that entry into several local segments, of
+// which accum_optical_depth.c:93-101 keeps the last).  Returns the number of
+// segments the reference would form (for the NSEG_PER_RAY check).
+__device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3f* start) {
+  const float dxd = (g.xmax - g.xmin) / 1.0f;
+  const float dyd = (g.ymax - g.ymin) / 1.0f;
+  const float dzd = (g.zmax - g.zmin) / 1.0f;
+  const int ixt = (int)((tgt.x - g.xmin) / dxd);
+  const int iyt = (int)((tgt.y - g.ymin) / dyd);
+  const int izt = (int)((tgt.z - g.zmin) / dzd);
+  const bool inside = !(src.x < g.xmin || src.x > g.xmax || src.y < g.ymin || src.y > g.ymax ||
+                        src.z < g.zmin || src.z > g.zmax);
+  if (inside) {
+    const int ixs = (int)((src.x - g.xmin) / dxd);
+    const int iys = (int)((src.y - g.ymin) / dyd);
+    const int izs = (int)((src.z - g.zmin) / dzd);
+    if (ixs == ixt && iys == iyt && izs == izt) {
+      *start = src;
+      return 1;
+    }
+  }
+  float dx = tgt.x - src.x, dy = tgt.y - src.y, dz = tgt.z - src.z;
+  float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kSegTiny);
+  float cos_t = dz / dl;
+  float sin_t = (float)__builtin_sqrt(1.0 - (double)(cos_t * cos_t));
+  float dxy = dl * sin_t;
+  float cos_p = (float)((double)dx / ((double)dxy + kSegTiny));
+  float sin_p = (float)((double)dy / ((double)dxy + kSegTiny));
+  float xo = cos_p * sin_t;
+  if ((double)fabsf(xo) < kSegTiny) xo = (float)((double)xo >= 0.0 ? kSegTiny : -kSegTiny);
+  float yo = sin_p * sin_t;
+  if (fabs((double)yo) < kSegTiny) yo = (float)((double)yo >= 0.0 ? kSegTiny : -kSegTiny);
+  float zo = cos_t;
+  if (fabs((double)zo) < kSegTiny) zo = (float)((double)zo >= 0.0 ? kSegTiny : -kSegTiny);
+  const int xs = (double)xo > 0.0, ys = (double)yo > 0.0, zs = (double)zo > 0.0;
+  const int xn = xs ? 1 : -1, yn = ys ? 1 : -1, zn = zs ? 1 : -1;
+  float out = 0.0f;
+  if (src.x < g.xmin) out = (g.xmin - src.x) / xo;
+  if (src.x > g.xmax) out = (g.xmax - src.x) / xo;
+  if (src.y < g.ymin) out = (g.ymin - src.y) / yo;
+  if (src.y > g.ymax) out = (g.ymax - src.y) / yo;
+  if (src.z < g.zmin) out = (g.zmin - src.z) / zo;
+  if (src.z > g.zmax) out = (g.zmax - src.z) / zo;
+  float sx, sy, sz;
+  if ((double)out != 0.0) {
+    sx = src.x + out * xo;
+    sy = src.y + out * yo;
+    sz = src.z + out * zo;
+  } else {
+    sx = src.x;
+    sy = src.y;
+    sz = src.z;
+  }
+  int ixs = (int)((sx - g.xmin) / dxd);
+  int iys = (int)((sy - g.ymin) / dyd);
+  int izs = (int)((sz - g.zmin) / dzd);
+  int nseg = 1;
+  while ((ixs != ixt || iys != iyt || izs != izt) && nseg < kMaxSegments) {
+    float ddx = g.xmin + dxd * (float)(xs + ixs) - sx;
+    float ddy = g.ymin + dyd * (float)(ys + iys) - sy;
+    float ddz = g.zmin + dzd * (float)(zs + izs) - sz;
+    float rx = ddx / xo, ry = ddy / yo, rz = ddz / zo;
+    float rmin = fminf(rx, fminf(ry, rz));
+    if (rmin == rx) ixs += xn;
+    if (rmin == ry) iys += yn;
+    if (rmin == rz) izs += zn;
+    sx = sx + rmin * xo;
+    sy = sy + rmin * yo;
+    sz = sz + rmin * zo;
+    nseg++;
+  }
+  start->x = sx;
+  start->y = sy;
+  start->z = sz;
+  return nseg;
+}
+
+// Per-ray state of the DDA walk, calc_optical_depth.c:76-143.
+struct Dda {
+  int ix, iy, iz;        // current cell
+  int ixe, iye, ize;     // end cell (clamped)
+  int64_t idx;           // linear index of the current cell
+  float xo, yo, zo;      // direction cosines (with the TINY guards)
+  double rxo, ryo, rzo;  // 1/xo etc. in double (exact-division helper)
+  float xsf, ysf, zsf;   // x_step etc. as float (0 or 1)
+  int xn, yn, zn;        // +-1
+  float xc, yc, zc;      // position in local coordinates
+};
+
+__device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
+  int ixe = (int)((e.x - g.xmin) / g.dx);
+  int iye = (int)((e.y - g.ymin) / g.dy);
+  int ize = (int)((e.z - g.zmin) / g.dz);
+  d->ixe = max(min(ixe, g.nx - 1), 0);
+  d->iye = max(min(iye, g.ny - 1), 0);
+  d->ize = max(min(ize, g.nz - 1), 0);
+  int ix = (int)((s.x - g.xmin) / g.dx);
+  int iy = (int)((s.y - g.ymin) / g.dy);
+  int iz = (int)((s.z - g.zmin) / g.dz);
+  d->ix = max(min(ix, g.nx - 1), 0);
+  d->iy = max(min(iy, g.ny - 1), 0);
+  d->iz = max(min(iz, g.nz - 1), 0);
+  d->idx = (int64_t)d->iz + (int64_t)g.nz * ((int64_t)d->iy + (int64_t)g.ny * d->ix);
+
+  float dx = e.x - s.x, dy = e.y - s.y, dz = e.z - s.z;
+  float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kWalkTiny);
+  float dxy = (float)__builtin_sqrt((double)(dx * dx + dy * dy));
+  float cos_t = dz / dl;
+  float sin_t = dxy / dl;
+  float cos_p = (float)((double)dx / ((double)dxy + kWalkTiny));
+  float sin_p = (float)((double)dy / ((double)dxy + kWalkTiny));
+  float xo = cos_p * sin_t;
+  if ((double)fabsf(xo) < kWalkTiny) xo = (float)((double)xo >= 0.0 ? kWalkTiny : -kWalkTiny);
+  float yo = sin_p * sin_t;
+  if (fabs((double)yo) < kWalkTiny) yo = (float)((double)yo >= 0.0 ? kWalkTiny : -kWalkTiny);
+  float zo = cos_t;
+  if (fabs((double)zo) < kWalkTiny) zo = (float)((double)zo >= 0.0 ? kWalkTiny : -kWalkTiny);
+  d->xo = xo;
+  d->yo = yo;
+  d->zo = zo;
+  d->rxo = 1.0 / (double)xo;
+  d->ryo = 1.0 / (double)yo;
+  d->rzo = 1.0 / (double)zo;
+  const bool px = (double)xo > 0.0, py = (double)yo > 0.0, pz = (double)zo > 0.0;
+  d->xsf = px ? 1.0f : 0.0f;
+  d->ysf = py ? 1.0f : 0.0f;
+  d->zsf = pz ? 1.0f : 0.0f;
+  d->xn = px ? 1 : -1;
+  d->yn = py ? 1 : -1;
+  d->zn = pz ? 1 : -1;
+  d->xc = s.x - g.xmin;
+  d->yc = s.y - g.ymin;
+  d->zc = s.z - g.zmin;
+}
+
+__device__ __forceinline__ bool dda_active(const GridDev& g, const Dda& d) {
+  return (d.ix != d.ixe || d.iy != d.iye || d.iz != d.ize) && (0 <= d.ix && d.ix < g.nx) &&
+         (0 <= d.iy && d.iy < g.ny) && (0 <= d.iz && d.iz < g.nz);
+}
+
+// One step of calc_optical_depth.c:163-179; returns the chord length.
+__device__ __forceinline__ float dda_step(const GridDev& g, Dda* d) {
+  const float ddx = g.dx * ((float)d->ix + d->xsf) - d->xc;
+  const float ddy = g.dy * ((float)d->iy + d->ysf) - d->yc;
+  const float ddz = g.dz * ((float)d->iz + d->zsf) - d->zc;
+  const float rx = div_by(ddx, d->rxo);
+  const float ry = div_by(ddy, d->ryo);
+  const float rz = div_by(ddz, d->rzo);
+  const float rmin = fminf(rx, fminf(ry, rz));
+  if (rmin == rx) {
+    d->ix += d->xn;
+    d->idx += (int64_t)d->xn * g.nyz;
+  }
+  if (rmin == ry) {
+    d->iy += d->yn;
+    d->idx += (int64_t)d->yn * g.nz;
+  }
+  if (rmin == rz) {
+    d->iz += d->zn;
+    d->idx += d->zn;
+  }
+  d->xc = d->xc + rmin * d->xo;
+  d->yc = d->yc + rmin * d->yo;
+  d->zc = d->zc + rmin * d->zo;
+  return rmin;
+}
+
+// calc_optical_depth.c:194-203: length from the last face to the end point.
+__device__ __forceinline__ float dda_depth(const GridDev& g, const Dda& d, Vec3f e) {
+  const float x = d.xc + g.xmin, y = d.yc + g.ymin, z = d.zc + g.zmin;
+  const float ex = e.x - x, ey = e.y - y, ez = e.z - z;
+  return (float)__builtin_sqrt((double)(ex * ex + ey * ey + ez * ez));
+}
+
+struct WalkOut {
+  float tau[3];
+  int ncross;
+  uint64_t hash;
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float term) {
+  const float y = term - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+// The whole walk of one segment (calc_optical_depth.c:76-227).  nf holds the
+// per-cell FP32 densities (nHI, nHeI, nHeII) formed as the reference forms
+// them.  KAHAN selects compensated sums; HASH records the cell sequence
+// (FNV-1a over the linear cell indices, the end cell last).
+template <bool KAHAN, bool HASH>
+__device__ inline void walk_segment(const GridDev& g, const float4* __restrict__ nf, Vec3f s,
+                                    Vec3f e, WalkOut* out) {
+  Dda d;
+  dda_setup(g, s, e, &d);
+  float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
+  float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  int nc = 0;
+  uint64_t h = 0xcbf29ce484222325ull;
+  while (dda_active(g, d)) {
+    const float4 n = nf[d.idx];
+    if (HASH) h = (h ^ (uint64_t)(uint32_t)d.idx) * 0x100000001b3ull;
+    nc++;
+    const float rmin = dda_step(g, &d);
+    if (KAHAN) {
+      kahan_add(t0, c0, rmin * n.x);
+      kahan_add(t1, c1, rmin * n.y);
+      kahan_add(t2, c2, rmin * n.z);
+    } else {
+      t0 = t0 + rmin * n.x;
+      t1 = t1 + rmin * n.y;
+      t2 = t2 + rmin * n.z;
+    }
+  }
+  const float depth = dda_depth(g, d, e);
+  const int64_t ie = (int64_t)d.ize + (int64_t)g.nz * ((int64_t)d.iye + (int64_t)g.ny * d.ixe);
+  const float4 n = nf[ie];
+  if (HASH) h = (h ^ (uint64_t)(uint32_t)ie) * 0x100000001b3ull;
+  nc++;
+  if (KAHAN) {
+    kahan_add(t0, c0, depth * n.x);
+    kahan_add(t1, c1, depth * n.y);
+    kahan_add(t2, c2, depth * n.z);
+  } else {
+    t0 = t0 + depth * n.x;
+    t1 = t1 + depth * n.y;
+    t2 = t2 + depth * n.z;
+  }
+  out->tau[0] = t0;
+  out->tau[1] = t1;
+  out->tau[2] = t2;
+  out->ncross = nc;
+  out->hash = h;
+}
+
+// delta_length, calc_photoion_rate.c:52-90.
+__device__ inline float delta_length(const GridDev& g, float dx, float dy, float dz) {
+  float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kRateTiny);
+  float dxy = (float)__builtin_sqrt((double)(dx * dx + dy * dy) + kRateTiny);
+  float cos_t = dz / dl;
+  float sin_t = dxy / dl;
+  float cos_p = dx / dxy;
+  float sin_p = dy / dxy;
+  float xo = cos_p * sin_t;
+  if ((double)fabsf(xo) < kRateTiny) xo = (float)((double)xo >= 0.0 ? kRateTiny : -kRateTiny);
+  float yo = sin_p * sin_t;
+  if ((double)fabsf(yo) < kRateTiny) yo = (float)((double)yo >= 0.0 ? kRateTiny : -kRateTiny);
+  float zo = cos_t;
+  if ((double)fabsf(zo) < kRateTiny) zo = (float)((double)zo >= 0.0 ? kRateTiny : -kRateTiny);
+  const float rx = fabsf(g.dx / xo);
+  const float ry = fabsf(g.dy / yo);
+  const float rz = fabsf(g.dz / zo);
+  return fminf(rx, fminf(ry, rz));
+}
+
+// Per-ray quantities of calc_photoion_rate.c:159-246 that do not depend on
+// frequency.
+struct RayRate {
+  double t[3];    // column densities at the cell entry [cm^-2]
+  double d[3];    // in-cell optical-depth weights n_s*delta*lunit
+  double r[3];    // 1/(n_s V_sh + TINY)
+};
+
+__device__ inline void ray_rate_setup(const GridDev& g, const RateTables& rt, const float tau[3],
+                                      Vec3f src, Vec3f tgt, float dens, float fHI, float fHeI,
+                                      float fHeII, RayRate* rr) {
+  const double col = g.lunit * g.denstonh;
+  rr->t[0] = (double)tau[0] * col;
+  rr->t[1] = (double)tau[1] * col;
+  rr->t[2] = (double)tau[2] * col;
+  const float dx = tgt.x - src.x, dy = tgt.y - src.y, dz = tgt.z - src.z;
+  const float delta = delta_length(g, dx, dy, dz);
+  const float dist = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz));
+  const double nH = (double)dens * g.denstonh;
+  const double nHI = nH * (double)fHI;
+  rr->d[0] = nHI * (double)delta * g.lunit;
+  rr->t[0] -= 0.5 * rr->d[0];
+  const double nHe = nH * g.helium_fact;
+  const double nHeI = nHe * (double)fHeI;
+  const double nHeII = nHe * (double)fHeII;
+  rr->d[1] = nHeI * (double)delta * g.lunit;
+  rr->d[2] = nHeII * (double)delta * g.lunit;
+  rr->t[1] -= 0.5 * rr->d[1];
+  rr->t[2] -= 0.5 * rr->d[2];
+  const double a = (double)dist + 0.5 * (double)delta;
+  const double b = (double)dist - 0.5 * (double)delta;
+  const double lu = g.lunit;
+  const double vsh = 4.0 * kPi / 3.0 * (a * a * a - b * b * b) * (lu * lu * lu);
+  rr->r[0] = 1.0 / (nHI * vsh + kRateTiny);
+  rr->r[1] = 1.0 / (nHeI * vsh + kRateTiny);
+  rr->r[2] = 1.0 / (nHeII * vsh + kRateTiny);
+}
+
+// The reference's frequency loop in FP64, calc_photoion_rate.c:285-387,
+// bin by bin (used when FP32 would overflow, e.g. a target cell that holds
+// the ray's source, where the reference itself produces inf/NaN).
+template <typename SpecFn>
+__device__ inline void rate_loop_f64(const RateTables& rt, const RayRate& rr, SpecFn spec,
+                                     double out[6]) {
+  double g0 = 0, h0 = 0, g1 = 0, g2 = 0, h1 = 0, h2 = 0;
+  for (int k = 0; k < rt.ngrid; k++) {
+    const float nu = rt.nu[k];
+    const double s0 = rt.cs[3 * k + 0], s1 = rt.cs[3 * k + 1], s2 = rt.cs[3 * k + 2];
+    double tau = rr.t[0] * s0;
+    double w0 = rr.d[0] * s0;
+    double dtot = w0;
+    tau += rr.t[1] * s1;
+    tau += rr.t[2] * s2;
+    double w1 = rr.d[1] * s1;
+    double w2 = rr.d[2] * s2;
+    dtot += w1 + w2;
+    const double absorbed = spec(k) * exp(-tau) * (1.0e0 - exp(-dtot));
+    w0 /= (dtot + kRateTiny);
+    g0 += w0 * absorbed * rr.r[0];
+    if ((double)nu > 1.0) h0 += w0 * absorbed * ((double)nu - 1.0) * rr.r[0];
+    w1 /= (dtot + kRateTiny);
+    w2 /= (dtot + kRateTiny);
+    g1 += w1 * absorbed * rr.r[1];
+    g2 += w2 * absorbed * rr.r[2];
+    if ((double)nu > 1.8125) h1 += w1 * absorbed * ((double)nu - 1.8125) * rr.r[1];
+    if ((double)nu > 4.0) h2 += w2 * absorbed * ((double)nu - 4.0) * rr.r[2];
+  }
+  out[0] = g0;
+  out[1] = h0 * kHnuL;
+  out[2] = g1;
+  out[3] = g2;
+  out[4] = h1 * kHnuL;
+  out[5] = h2 * kHnuL;
+}
+
+// (1 - exp(-x))/x in FP32: series below 0.25 (8 terms, truncation < 1e-9),
+// the direct form above.
+__device__ __forceinline__ float one_minus_exp_over(float x) {
+  const float p = 1.0f + x * (-1.0f / 2 + x * (1.0f / 6 + x * (-1.0f / 24 + x * (1.0f / 120 +
+                  x * (-1.0f / 720 + x * (1.0f / 5040 + x * (-1.0f / 40320)))))));
+  const float q = (1.0f - __expf(-x)) * __builtin_amdgcn_rcpf(x);
+  return x < 0.25f ? p : q;
+}
+
+}  // namespace argot

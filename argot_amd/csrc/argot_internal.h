@@ -1,0 +1,90 @@
+// Internal types shared by the host runtime and the HIP kernels of
+// libargot_hip.so.  Not part of the C-ABI (include/argot_hip.h is).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define ARGOT_HD __host__ __device__
+#else
+#define ARGOT_HD
+struct float4_host {
+  float x, y, z, w;
+};
+#endif
+
+namespace argot {
+
+constexpr int kMaxSegments = 5;   // NSEG_PER_RAY (run_param.h:60)
+constexpr int kNgridMax = 256;    // NGRID_NU (run_param.h:78)
+constexpr int kTile = 4;          // 4x4x4 target cells per wavefront
+constexpr int kNumQueues = 8;     // one tile queue per XCD
+
+// Mesh geometry and physical units as the kernels use them.
+struct GridDev {
+  int nx, ny, nz;                            // local mesh
+  int64_t nyz;
+  float xmin, ymin, zmin, xmax, ymax, zmax;  // local box (walk, calc_optical_depth.c:76-143)
+  float gxmin, gymin, gzmin;                 // global origin (target centres)
+  int ox, oy, oz;                            // global index of local cell 0
+  float dx, dy, dz;
+  double lunit, denstonh, helium_fact;
+  float theta_sq;   // SQR(theta_crit) in FP32 (setup_light_ray_tree.c:577)
+};
+
+// Frequency tables (device pointers).  sig/hsig are [3][ngrid] SoA:
+// sig[s][k] = sigma_s(nu_k), hsig[s][k] = sigma_s(nu_k) (nu_k - nu_s) for
+// nu_k > nu_s else 0, with nu_k the FP32 grid value as the reference uses.
+struct RateTables {
+  int ngrid;
+  const float* nu;     // [ngrid]
+  const float* cs;     // [3*ngrid], bin-major as run_param.csect
+  const float* sig;    // [3*ngrid], species-major
+  const float* hsig;   // [3*ngrid], species-major
+  double cs_max[3];    // max over bins of sigma_s
+  double cs_last[3];   // sigma_s at the last bin
+};
+
+// Source tree in depth-first pre-order (children in octant order), as
+// setup_light_ray_to (setup_light_ray_tree.c:321-395) visits it.
+struct NodeLink {
+  int skip;       // next node after this subtree (== nnode at the end)
+  int n;          // number of sources below
+  int first;      // first child (== self+1) or -1 when there is none
+  int src;        // for n == 1: the source (original index), else -1
+};
+
+struct TreeDev {
+  int nnode;
+  const float* node_cm;      // [4*nnode]: cm x, y, z and SQR(length)
+  const NodeLink* link;      // [nnode]
+  const float* spec;         // [nnode*ngrid] spectrum / spec_scale, FP32
+  const double* spec_scale;  // [nnode] max over bins of the spectrum
+  const double* spec64;      // [nnode*ngrid] the spectrum itself
+  const float* src_pos;      // [4*nsrc]
+};
+
+// One emitted ray, written only in capture mode (tests).
+struct RayRecord {
+  int64_t target;       // linear target index
+  int32_t seq;          // position in the target's ray list
+  int32_t kind;         // 0: tree node, 1: source
+  int32_t id;           // node (pre-order) or source index
+  int32_t ncross;       // crossings incl. the end cell
+  float start[3];       // segment start (global coordinates)
+  float src[3];         // ray source position
+  float tau[3];         // FP32 optical depths (code units)
+  float pad;
+  uint64_t hash;        // FNV-1a of the visited cell sequence
+  double rate[6];       // this ray's rates (heats times hplanck*nuL)
+};
+
+struct StepCounters {
+  unsigned long long nray;
+  unsigned long long ncross;
+  unsigned long long nfallback;
+  unsigned long long nrecord;
+  int queue[kNumQueues];
+};
+
+}  // namespace argot

@@ -1,0 +1,36 @@
+// Launch interface of kernels.hip (host side of libargot_hip.so).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "argot_internal.h"
+
+namespace argot {
+
+struct FusedArgs {
+  GridDev grid;
+  TreeDev tree;
+  RateTables rates;
+  const float* dsig;           // [3*ngrid] sigma_s(k) - sigma_s(last), species-major
+  const float4* nf;            // per-cell (nHI, nHeI, nHeII, 0)
+  unsigned char* mesh;         // AoS fluid_mesh (reads fractions, adds rates)
+  StepCounters* ctr;
+  RayRecord* rec;              // capture mode only
+  int64_t rec_cap;
+};
+
+hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fact, float4* nf,
+                              hipStream_t s);
+hipError_t launch_zero_rates(void* mesh, int64_t ncell, hipStream_t s);
+hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s);
+hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
+                                int kahan, hipStream_t s);
+hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const float* dsig,
+                            const void* mesh, const void* rays, int64_t nray, double* per_ray,
+                            int64_t* target, hipStream_t s);
+hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
+                                  void* mesh, hipStream_t s);
+hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s);
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int nblock,
+                             hipStream_t s);
+
+}  // namespace argot

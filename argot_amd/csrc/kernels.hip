@@ -1,0 +1,417 @@
+// HIP kernels of the point-source radiation step for gfx950 (MI355X).
+//
+//   pack_fields         AoS fluid_mesh -> per-cell FP32 (nHI, nHeI, nHeII)
+//   zero_rates          prev_chem rates := 0         (zero_out_photoion_rate)
+//   add_rates           prev_chem rates += table     (merge_photoion_data)
+//   walk_segments       optical depth of a segment list (calc_optical_depth)
+//   ray_rates           per-ray rates of a light_ray_IO list (calc_photoion_rate)
+//   merge_ray_rates     per-target sums of consecutive rays, no atomics
+//   fused_step          tree walk + segment + optical depth + rates + per-cell
+//                       sum for 4x4x4 target tiles, one wavefront per tile
+#include <hip/hip_runtime.h>
+
+#include "argot_device.h"
+#include "argot_internal.h"
+#include "kernels.h"
+
+namespace argot {
+
+// Offsets inside struct fluid_mesh (156 B, include/argot_hip.h).
+constexpr int kMeshStride = 156;
+constexpr int kOffDens = 0;
+constexpr int kOffFHI = 52;
+constexpr int kOffFHeI = 60;
+constexpr int kOffFHeII = 64;
+constexpr int kOffRates = 124;   // prev_chem.GammaHI .. HeatHeII (6 floats)
+
+__device__ __forceinline__ const float* mesh_f(const unsigned char* mesh, int64_t c, int off) {
+  return reinterpret_cast<const float*>(mesh + c * kMeshStride + off);
+}
+__device__ __forceinline__ float* mesh_fw(unsigned char* mesh, int64_t c, int off) {
+  return reinterpret_cast<float*>(mesh + c * kMeshStride + off);
+}
+
+// calc_optical_depth.c:150-156: nHI = dens*fHI, nHe = dens*HELIUM_FACT in
+// double rounded to float, nHeI = nHe*fHeI, all FP32.
+__global__ void pack_fields_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
+                                   double helium_fact, float4* __restrict__ nf) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float nH = *mesh_f(mesh, c, kOffDens);
+  const float nHI = nH * *mesh_f(mesh, c, kOffFHI);
+  const float nHe = (float)((double)nH * helium_fact);
+  const float nHeI = nHe * *mesh_f(mesh, c, kOffFHeI);
+  const float nHeII = nHe * *mesh_f(mesh, c, kOffFHeII);
+  nf[c] = make_float4(nHI, nHeI, nHeII, 0.0f);
+}
+
+__global__ void zero_rates_kernel(unsigned char* __restrict__ mesh, int64_t ncell) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  float* r = mesh_fw(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) r[k] = 0.0f;
+}
+
+__global__ void add_rates_kernel(unsigned char* __restrict__ mesh, const float* __restrict__ rate,
+                                 int64_t ncell) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  float* r = mesh_fw(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) r[k] = r[k] + rate[6 * c + k];
+}
+
+// ray_segment records (56 B): start @16, end @28, tau @40.
+__global__ void walk_segments_kernel(GridDev g, const float4* __restrict__ nf,
+                                     unsigned char* __restrict__ seg, int64_t nseg, int kahan) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  float* rec = reinterpret_cast<float*>(seg + i * 56);
+  const Vec3f s{rec[4], rec[5], rec[6]};
+  const Vec3f e{rec[7], rec[8], rec[9]};
+  WalkOut w;
+  if (kahan)
+    walk_segment<true, false>(g, nf, s, e, &w);
+  else
+    walk_segment<false, false>(g, nf, s, e, &w);
+  rec[10] = w.tau[0];
+  rec[11] = w.tau[1];
+  rec[12] = w.tau[2];
+}
+
+// FP32 frequency loop (photon-conserving form, calc_photoion_rate.c:327-386
+// regrouped): with B_k = N_k exp(-tau_k) (1 - exp(-dtau_k)) / dtau_k,
+//   Gamma_s = r_s d_s sum_k sigma_s(k) B_k,  Heat_s = r_s d_s sum_k hsig_s(k) B_k.
+// The optical depth is shifted by its last-bin value tsh (folded back in
+// FP64), sums run in FP32 over 32 bins and in FP64 across chunks.
+template <typename SpecF>
+__device__ inline void rate_loop_f32(const RateTables& rt, const float* __restrict__ dsig,
+                                     const RayRate& rr, double scale, SpecF spec,
+                                     double out[6]) {
+  const int ng = rt.ngrid;
+  const double tsh = rr.t[0] * rt.cs_last[0] + rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2];
+  const float n0 = (float)rr.t[0], n1 = (float)rr.t[1], n2 = (float)rr.t[2];
+  const float d0 = (float)rr.d[0], d1 = (float)rr.d[1], d2 = (float)rr.d[2];
+  double G0 = 0, G1 = 0, G2 = 0, H0 = 0, H1 = 0, H2 = 0;
+  for (int c = 0; c < ng; c += 32) {
+    float g0 = 0, g1 = 0, g2 = 0, h0 = 0, h1 = 0, h2 = 0;
+    const int ce = min(c + 32, ng);
+    for (int k = c; k < ce; k++) {
+      const float s0 = rt.sig[k], s1 = rt.sig[ng + k], s2 = rt.sig[2 * ng + k];
+      const float tau = __builtin_fmaf(n2, dsig[2 * ng + k],
+                                       __builtin_fmaf(n1, dsig[ng + k], n0 * dsig[k]));
+      const float dt = __builtin_fmaf(d2, s2, __builtin_fmaf(d1, s1, d0 * s0));
+      const float b = spec(k) * __expf(-tau) * one_minus_exp_over(dt);
+      g0 = __builtin_fmaf(s0, b, g0);
+      g1 = __builtin_fmaf(s1, b, g1);
+      g2 = __builtin_fmaf(s2, b, g2);
+      h0 = __builtin_fmaf(rt.hsig[k], b, h0);
+      h1 = __builtin_fmaf(rt.hsig[ng + k], b, h1);
+      h2 = __builtin_fmaf(rt.hsig[2 * ng + k], b, h2);
+    }
+    G0 += g0; G1 += g1; G2 += g2;
+    H0 += h0; H1 += h1; H2 += h2;
+  }
+  const double pre = scale * exp(-tsh);
+  const double p0 = rr.r[0] * rr.d[0] * pre;
+  const double p1 = rr.r[1] * rr.d[1] * pre;
+  const double p2 = rr.r[2] * rr.d[2] * pre;
+  out[0] = p0 * G0;
+  out[1] = p0 * H0 * kHnuL;
+  out[2] = p1 * G1;
+  out[3] = p2 * G2;
+  out[4] = p1 * H1 * kHnuL;
+  out[5] = p2 * H2 * kHnuL;
+}
+
+// Lower bound of the shifted bin optical depth and upper bound of the
+// in-cell optical depth decide whether the FP32 loop is safe.
+__device__ __forceinline__ bool fast_rate_ok(const RateTables& rt, const RayRate& rr) {
+  double lo = 0.0, dmax = 0.0;
+  bool finite = true;
+  for (int s = 0; s < 3; s++) {
+    lo += rr.t[s] >= 0.0 ? -rr.t[s] * rt.cs_last[s] : rr.t[s] * (rt.cs_max[s] - rt.cs_last[s]);
+    dmax += rr.d[s] * rt.cs_max[s];
+    finite = finite && fabs(rr.t[s]) < 1.0e36 && rr.d[s] < 1.0e36;
+  }
+  return finite && lo > -60.0 && dmax < 1.0e30;
+}
+
+// Legacy per-ray rate kernel: one thread per light_ray_IO record (2096 B:
+// src@0 {type, param, xpos@8, ypos, zpos, photon_rate@24}, ix_target@2072,
+// tau@2084).  Per-ray results go to out[6*i] for the segmented merge.
+__global__ void ray_rates_kernel(GridDev g, RateTables rt, const float* __restrict__ dsig,
+                                 const unsigned char* __restrict__ mesh,
+                                 const unsigned char* __restrict__ rays, int64_t nray,
+                                 double* __restrict__ out, int64_t* __restrict__ target) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nray) return;
+  const unsigned char* r = rays + i * 2096;
+  const float* rf = reinterpret_cast<const float*>(r);
+  const double* spec = reinterpret_cast<const double*>(r + 24);
+  const int* it = reinterpret_cast<const int*>(r + 2072);
+  const Vec3f src{rf[2], rf[3], rf[4]};
+  // global target indices (calc_photoion_rate.c:176-184): local cell for
+  // the mesh, global origin for the position
+  const int ix = it[0], iy = it[1], iz = it[2];
+  const float* tau = reinterpret_cast<const float*>(r + 2084);
+  const int lx = ix - g.ox, ly = iy - g.oy, lz = iz - g.oz;
+  const int64_t c = (int64_t)lz + (int64_t)g.nz * ((int64_t)ly + (int64_t)g.ny * lx);
+  target[i] = c;
+  const Vec3f tgt{cell_centre(g.gxmin, ix, g.dx), cell_centre(g.gymin, iy, g.dy),
+                  cell_centre(g.gzmin, iz, g.dz)};
+  RayRate rr;
+  ray_rate_setup(g, rt, tau, src, tgt, *mesh_f(mesh, c, kOffDens), *mesh_f(mesh, c, kOffFHI),
+                 *mesh_f(mesh, c, kOffFHeI), *mesh_f(mesh, c, kOffFHeII), &rr);
+  double res[6];
+  if (fast_rate_ok(rt, rr)) {
+    double smax = 0.0;
+    for (int k = 0; k < rt.ngrid; k++) smax = fmax(smax, spec[k]);
+    const double inv = smax > 0.0 ? 1.0 / smax : 0.0;
+    rate_loop_f32(rt, dsig, rr, smax, [&](int k) { return (float)(spec[k] * inv); }, res);
+  } else {
+    rate_loop_f64(rt, rr, [&](int k) { return spec[k]; }, res);
+  }
+  for (int k = 0; k < 6; k++) out[6 * i + k] = res[k];
+}
+
+// Rays of one target are consecutive (setup_light_ray_range emits them in
+// target order): the thread owning the first ray of each run sums the run
+// in ray order in FP64 and adds it to the cell once.
+__global__ void merge_ray_rates_kernel(const double* __restrict__ per_ray,
+                                       const int64_t* __restrict__ target, int64_t nray,
+                                       unsigned char* __restrict__ mesh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nray) return;
+  const int64_t c = target[i];
+  if (i > 0 && target[i - 1] == c) return;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t j = i; j < nray && target[j] == c; j++)
+    for (int k = 0; k < 6; k++) acc[k] += per_ray[6 * j + k];
+  float* r = mesh_fw(mesh, c, kOffRates);
+  for (int k = 0; k < 6; k++) r[k] = (float)((double)r[k] + acc[k]);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Next tile for this wavefront: its own XCD's queue first (blocks are
+// dispatched round-robin over the 8 XCDs), then the others in turn.
+__device__ __forceinline__ int64_t next_tile(StepCounters* ctr, int64_t ntile, int* q, int* tried) {
+  while (*tried < kNumQueues) {
+    const int64_t lo = ntile * (*q) / kNumQueues;
+    const int64_t hi = ntile * (*q + 1) / kNumQueues;
+    int t = 0;
+    if (threadIdx.x == 0) t = atomicAdd(&ctr->queue[*q], 1);
+    t = __shfl(t, 0, 64);
+    if (lo + t < hi) return lo + t;
+    *q = (*q + 1) % kNumQueues;
+    (*tried)++;
+  }
+  return -1;
+}
+
+template <bool KAHAN, bool CAPTURE>
+__global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs a) {
+  const GridDev& g = a.grid;
+  const TreeDev& tr = a.tree;
+  const RateTables& rt = a.rates;
+  const int lane = threadIdx.x;
+  const int ntx = (g.nx + kTile - 1) / kTile, nty = (g.ny + kTile - 1) / kTile,
+            ntz = (g.nz + kTile - 1) / kTile;
+  const int64_t ntile = (int64_t)ntx * nty * ntz;
+  int q = blockIdx.x % kNumQueues, tried = 0;
+  unsigned long long my_ray = 0, my_cross = 0, my_fb = 0;
+
+  for (;;) {
+    const int64_t t = next_tile(a.ctr, ntile, &q, &tried);
+    if (t < 0) break;
+    const int tz = (int)(t % ntz);
+    const int ty = (int)((t / ntz) % nty);
+    const int tx = (int)(t / ((int64_t)ntz * nty));
+    const int ix = tx * kTile + ((lane >> 4) & 3);
+    const int iy = ty * kTile + ((lane >> 2) & 3);
+    const int iz = tz * kTile + (lane & 3);
+    const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    const int64_t cell = (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
+    const Vec3f tgt{cell_centre(g.gxmin, ix + g.ox, g.dx), cell_centre(g.gymin, iy + g.oy, g.dy),
+                    cell_centre(g.gzmin, iz + g.oz, g.dz)};
+    float dens = 0, fHI = 0, fHeI = 0, fHeII = 0;
+    if (valid) {
+      dens = *mesh_f(a.mesh, cell, kOffDens);
+      fHI = *mesh_f(a.mesh, cell, kOffFHI);
+      fHeI = *mesh_f(a.mesh, cell, kOffFHeI);
+      fHeII = *mesh_f(a.mesh, cell, kOffFHeII);
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int p = valid ? 0 : tr.nnode;
+    int seq = 0;
+    for (;;) {
+      const int u = wave_min(p);
+      if (u >= tr.nnode) break;
+      const float cx = tr.node_cm[4 * u + 0], cy = tr.node_cm[4 * u + 1],
+                  cz = tr.node_cm[4 * u + 2], len2 = tr.node_cm[4 * u + 3];
+      const NodeLink ln = tr.link[u];
+      const bool here = (p == u);
+      // setup_light_ray_to (setup_light_ray_tree.c:346-354), FP32
+      const float ddx = tgt.x - cx, ddy = tgt.y - cy, ddz = tgt.z - cz;
+      const float r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      const bool accept = r2 * g.theta_sq > len2;
+      const bool leaf = ln.n <= 1;
+      const bool emit = here && (accept || ln.n == 1);
+      if (here) p = (accept || leaf || ln.first < 0) ? ln.skip : ln.first;
+      if (emit) {
+        Vec3f src;
+        if (accept) {
+          src = Vec3f{cx, cy, cz};
+        } else {
+          src = Vec3f{tr.src_pos[4 * ln.src + 0], tr.src_pos[4 * ln.src + 1],
+                      tr.src_pos[4 * ln.src + 2]};
+        }
+        Vec3f st;
+        segment_start(g, src, tgt, &st);
+        WalkOut w;
+        walk_segment<KAHAN, CAPTURE>(g, a.nf, st, tgt, &w);
+        RayRate rr;
+        ray_rate_setup(g, rt, w.tau, src, tgt, dens, fHI, fHeI, fHeII, &rr);
+        double res[6];
+        if (fast_rate_ok(rt, rr)) {
+          const float* sp = tr.spec + (int64_t)u * rt.ngrid;
+          rate_loop_f32(rt, a.dsig, rr, tr.spec_scale[u], [&](int k) { return sp[k]; }, res);
+        } else {
+          const double* sp = tr.spec64 + (int64_t)u * rt.ngrid;
+          rate_loop_f64(rt, rr, [&](int k) { return sp[k]; }, res);
+          my_fb++;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] += res[k];
+        my_ray++;
+        my_cross += (unsigned long long)w.ncross;
+        if (CAPTURE) {
+          const unsigned long long slot = atomicAdd(&a.ctr->nrecord, 1ull);
+          if (slot < (unsigned long long)a.rec_cap) {
+            RayRecord& R = a.rec[slot];
+            R.target = cell;
+            R.seq = seq;
+            R.kind = accept ? 0 : 1;
+            R.id = accept ? u : ln.src;
+            R.ncross = w.ncross;
+            R.start[0] = st.x; R.start[1] = st.y; R.start[2] = st.z;
+            R.src[0] = src.x; R.src[1] = src.y; R.src[2] = src.z;
+            R.tau[0] = w.tau[0]; R.tau[1] = w.tau[1]; R.tau[2] = w.tau[2];
+            R.pad = 0.0f;
+            R.hash = w.hash;
+            for (int k = 0; k < 6; k++) R.rate[k] = res[k];
+          }
+        }
+        seq++;
+      }
+    }
+    if (valid) {
+      float* r = mesh_fw(a.mesh, cell, kOffRates);
+#pragma unroll
+      for (int k = 0; k < 6; k++) r[k] = (float)((double)r[k] + acc[k]);
+    }
+  }
+  const unsigned long long sr = wave_sum(my_ray), sc = wave_sum(my_cross),
+                           sf = wave_sum(my_fb);
+  if (lane == 0) {
+    atomicAdd(&a.ctr->nray, sr);
+    atomicAdd(&a.ctr->ncross, sc);
+    atomicAdd(&a.ctr->nfallback, sf);
+  }
+}
+
+__global__ void reset_counters_kernel(StepCounters* ctr) {
+  const int i = threadIdx.x;
+  if (i < kNumQueues) ctr->queue[i] = 0;
+  if (i == 0) {
+    ctr->nray = 0;
+    ctr->ncross = 0;
+    ctr->nfallback = 0;
+    ctr->nrecord = 0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Launch wrappers (host side).
+
+static inline unsigned nblocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fact, float4* nf,
+                              hipStream_t s) {
+  pack_fields_kernel<<<nblocks(ncell, 256), 256, 0, s>>>(
+      static_cast<const unsigned char*>(mesh), ncell, helium_fact, nf);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_rates(void* mesh, int64_t ncell, hipStream_t s) {
+  zero_rates_kernel<<<nblocks(ncell, 256), 256, 0, s>>>(static_cast<unsigned char*>(mesh), ncell);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s) {
+  add_rates_kernel<<<nblocks(ncell, 256), 256, 0, s>>>(static_cast<unsigned char*>(mesh), rate,
+                                                        ncell);
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
+                                int kahan, hipStream_t s) {
+  if (nseg <= 0) return hipSuccess;
+  walk_segments_kernel<<<nblocks(nseg, 256), 256, 0, s>>>(
+      g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const float* dsig,
+                            const void* mesh, const void* rays, int64_t nray, double* per_ray,
+                            int64_t* target, hipStream_t s) {
+  if (nray <= 0) return hipSuccess;
+  ray_rates_kernel<<<nblocks(nray, 128), 128, 0, s>>>(
+      g, rt, dsig, static_cast<const unsigned char*>(mesh),
+      static_cast<const unsigned char*>(rays), nray, per_ray, target);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
+                                  void* mesh, hipStream_t s) {
+  if (nray <= 0) return hipSuccess;
+  merge_ray_rates_kernel<<<nblocks(nray, 256), 256, 0, s>>>(per_ray, target, nray,
+                                                             static_cast<unsigned char*>(mesh));
+  return hipGetLastError();
+}
+
+hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s) {
+  reset_counters_kernel<<<1, 64, 0, s>>>(ctr);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int nblock,
+                             hipStream_t s) {
+  if (capture) {
+    if (kahan)
+      fused_step_kernel<true, true><<<nblock, 64, 0, s>>>(a);
+    else
+      fused_step_kernel<false, true><<<nblock, 64, 0, s>>>(a);
+  } else {
+    if (kahan)
+      fused_step_kernel<true, false><<<nblock, 64, 0, s>>>(a);
+    else
+      fused_step_kernel<false, false><<<nblock, 64, 0, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace argot

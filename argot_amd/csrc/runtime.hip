@@ -1,0 +1,621 @@
+// Host runtime of libargot_hip.so: the reference's GPU entry points
+// (src/prototype.h:86-130) on one MI355X, the fused step_radiation_tree,
+// and the library controls declared in include/argot_hip.h.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "argot_hip.h"
+#include "argot_internal.h"
+#include "kernels.h"
+#include "tree.h"
+
+using namespace argot;
+
+namespace {
+
+thread_local int t_err = ARGOT_OK;
+
+void set_err(int e) {
+  if (t_err == ARGOT_OK) t_err = e;
+}
+
+#define HIP_OK(expr)                                        \
+  do {                                                      \
+    hipError_t _e = (expr);                                 \
+    if (_e != hipSuccess) {                                 \
+      set_err(_e == hipErrorOutOfMemory ? ARGOT_ERR_NOMEM   \
+                                        : ARGOT_ERR_HIP);   \
+      return;                                               \
+    }                                                       \
+  } while (0)
+
+// Options (library controls).
+double g_helium_fact = 0.0811258278;   // chemistry.h:25 (Y = 0.245)
+int g_tau_mode = 0;
+int g_rate_mode = 0;
+float g_theta = 0.7f;                  // ARGOT_THETA_CRIT, run_param.h:80
+int g_blocks_per_cu = 16;
+RayRecord* g_capture_host = nullptr;
+int64_t g_capture_cap = 0;
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t reserve(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t want = std::max<size_t>(n, 1);
+    hipError_t e = hipMalloc(&p, want * sizeof(T));
+    if (e == hipSuccess) cap = want;
+    return e;
+  }
+};
+
+struct Ctx {
+  bool ready = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int ncu = 256;
+  int nx = 0, ny = 0, nz = 0;
+  int64_t ncell = 0;
+  unsigned char* mesh = nullptr;   // cuda_mem->mesh_dev
+  DevBuf<float4> nf;
+  // frequency tables
+  DevBuf<float> nu, cs, sig, hsig, dsig;
+  double cs_max[3] = {0, 0, 0}, cs_last[3] = {0, 0, 0};
+  int ngrid = 0;
+  // tree
+  DevBuf<NodeLink> link;
+  DevBuf<float> cm, spec, src_pos;
+  DevBuf<double> spec_scale, spec64;
+  DevBuf<StepCounters> ctr;
+  DevBuf<RayRecord> rec;
+  // segment / ray staging for the per-call entry points
+  DevBuf<unsigned char> seg, rays;
+  DevBuf<double> per_ray;
+  DevBuf<int64_t> target;
+  DevBuf<float> rate_tab;
+  HostTree tree;
+  argot_step_stats stats{};
+};
+
+Ctx g;
+
+GridDev make_grid(const run_param* r) {
+  GridDev d;
+  d.nx = r->nmesh_x_local;
+  d.ny = r->nmesh_y_local;
+  d.nz = r->nmesh_z_local;
+  d.nyz = (int64_t)d.ny * d.nz;
+  d.xmin = r->xmin_local;
+  d.ymin = r->ymin_local;
+  d.zmin = r->zmin_local;
+  d.xmax = r->xmax_local;
+  d.ymax = r->ymax_local;
+  d.zmax = r->zmax_local;
+  d.gxmin = r->xmin;
+  d.gymin = r->ymin;
+  d.gzmin = r->zmin;
+  d.ox = r->rank_x * r->nmesh_x_local;
+  d.oy = r->rank_y * r->nmesh_y_local;
+  d.oz = r->rank_z * r->nmesh_z_local;
+  d.dx = r->delta_x;
+  d.dy = r->delta_y;
+  d.dz = r->delta_z;
+  d.lunit = r->lunit;
+  d.denstonh = r->denstonh;
+  d.helium_fact = g_helium_fact;
+  const float th = r->theta_crit;
+  d.theta_sq = th * th;
+  return d;
+}
+
+RateTables rate_tables() {
+  RateTables t;
+  t.ngrid = g.ngrid;
+  t.nu = g.nu.p;
+  t.cs = g.cs.p;
+  t.sig = g.sig.p;
+  t.hsig = g.hsig.p;
+  for (int s = 0; s < 3; s++) {
+    t.cs_max[s] = g.cs_max[s];
+    t.cs_last[s] = g.cs_last[s];
+  }
+  return t;
+}
+
+// Tables of run_param (freq.nu, csect) in the layouts the kernels read.
+void upload_tables(const run_param* r) {
+  const int ng = ARGOT_NGRID_NU;
+  std::vector<float> nu(ng), cs(3 * ng), sig(3 * ng), hsig(3 * ng), dsig(3 * ng);
+  const double thr[3] = {1.0, 1.8125, 4.0};   // HI, HeI, HeII limits [nu_L]
+  for (int k = 0; k < ng; k++) {
+    nu[k] = (float)r->freq.nu[k];
+    const float c[3] = {r->csect[k].csect_HI, r->csect[k].csect_HeI, r->csect[k].csect_HeII};
+    for (int s = 0; s < 3; s++) {
+      cs[3 * k + s] = c[s];
+      sig[s * ng + k] = c[s];
+      hsig[s * ng + k] =
+          (double)nu[k] > thr[s] ? (float)((double)c[s] * ((double)nu[k] - thr[s])) : 0.0f;
+    }
+  }
+  for (int s = 0; s < 3; s++) {
+    double mx = 0.0;
+    for (int k = 0; k < ng; k++) mx = std::max(mx, (double)sig[s * ng + k]);
+    g.cs_max[s] = mx;
+    g.cs_last[s] = sig[s * ng + ng - 1];
+    for (int k = 0; k < ng; k++)
+      dsig[s * ng + k] = (float)((double)sig[s * ng + k] - g.cs_last[s]);
+  }
+  g.ngrid = ng;
+  HIP_OK(g.nu.reserve(ng));
+  HIP_OK(g.cs.reserve(3 * ng));
+  HIP_OK(g.sig.reserve(3 * ng));
+  HIP_OK(g.hsig.reserve(3 * ng));
+  HIP_OK(g.dsig.reserve(3 * ng));
+  HIP_OK(hipMemcpy(g.nu.p, nu.data(), ng * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(g.cs.p, cs.data(), 3 * ng * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(g.sig.p, sig.data(), 3 * ng * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(g.hsig.p, hsig.data(), 3 * ng * sizeof(float), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(g.dsig.p, dsig.data(), 3 * ng * sizeof(float), hipMemcpyHostToDevice));
+}
+
+bool check_ready(const cuda_mem_space* cm, const run_param* r) {
+  if (!g.ready || !cm || !cm->mesh_dev || !r) {
+    set_err(ARGOT_ERR_STATE);
+    return false;
+  }
+  if (r->nmesh_x_local != g.nx || r->nmesh_y_local != g.ny || r->nmesh_z_local != g.nz) {
+    set_err(ARGOT_ERR_ARG);
+    return false;
+  }
+  g.mesh = reinterpret_cast<unsigned char*>(cm->mesh_dev);
+  return true;
+}
+
+void pack_fields() {
+  HIP_OK(g.nf.reserve(g.ncell));
+  HIP_OK(launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream));
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int argot_hip_last_error(void) { return t_err; }
+
+const char* argot_hip_error_string(int code) {
+  switch (code) {
+    case ARGOT_OK: return "ok";
+    case ARGOT_ERR_HIP: return "HIP runtime call failed";
+    case ARGOT_ERR_ARG: return "bad argument or size";
+    case ARGOT_ERR_STATE: return "init_gpu / send_mesh_data not called";
+    case ARGOT_ERR_NOMEM: return "device allocation failed";
+    case ARGOT_ERR_TOO_MANY_SEGMENTS: return "ray crosses too many domains";
+    default: return "unknown";
+  }
+}
+
+void argot_hip_set_helium_fact(double helium_fact) { g_helium_fact = helium_fact; }
+void argot_hip_set_tau_mode(int mode) { g_tau_mode = mode ? 1 : 0; }
+void argot_hip_set_rate_mode(int mode) { g_rate_mode = mode ? 1 : 0; }
+void argot_hip_step_stats(struct argot_step_stats* out) {
+  if (out) *out = g.stats;
+}
+
+// Opening angle used by step_radiation_tree (the reference always sets
+// ARGOT_THETA_CRIT = 0.7, src/step_radiation_tree.c:594).
+void argot_hip_set_theta(float theta) { g_theta = theta; }
+
+// Persistent grid size of the fused kernel, in wavefronts per CU.
+void argot_hip_set_blocks_per_cu(int n) { g_blocks_per_cu = n > 0 ? n : 16; }
+
+// Test support: the next step_radiation_tree records every ray it forms
+// (struct RayRecord, 120 B) into host_records[0..cap).
+void argot_hip_set_capture(void* host_records, int64_t cap) {
+  g_capture_host = static_cast<RayRecord*>(host_records);
+  g_capture_cap = host_records ? cap : 0;
+}
+
+int64_t argot_hip_record_size(void) { return (int64_t)sizeof(RayRecord); }
+
+void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+              struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)mesh;
+  t_err = ARGOT_OK;
+  if (!cuda_mem || !this_cuda || !this_run) return set_err(ARGOT_ERR_ARG);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(ARGOT_ERR_HIP);
+  HIP_OK(hipSetDevice(0));
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDeviceProperties(&prop, 0));
+  g.ncu = prop.multiProcessorCount;
+  g.nx = this_run->nmesh_x_local;
+  g.ny = this_run->nmesh_y_local;
+  g.nz = this_run->nmesh_z_local;
+  if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0) return set_err(ARGOT_ERR_ARG);
+  g.ncell = (int64_t)g.nx * g.ny * g.nz;
+  memset(this_cuda, 0, sizeof(*this_cuda));
+  this_cuda->num_cuda_dev = 1;
+  this_cuda->nmesh_per_dev = (int)g.ncell;
+  this_cuda->cuda_nblock = (int)(g.ncell / 256);
+  this_cuda->max_thread_dimx = prop.maxThreadsDim[0];
+  this_cuda->max_block_dimx = prop.maxGridSize[0];
+  memset(cuda_mem, 0, sizeof(*cuda_mem));
+  HIP_OK(hipMalloc(&cuda_mem->mesh_dev, sizeof(struct fluid_mesh) * g.ncell));
+  HIP_OK(hipMalloc(&cuda_mem->this_run_dev, sizeof(struct run_param)));
+  HIP_OK(hipMalloc(&cuda_mem->rate_dev, sizeof(struct photoion_rate) * g.ncell));
+  HIP_OK(hipMemcpy(cuda_mem->this_run_dev, this_run, sizeof(struct run_param),
+                   hipMemcpyHostToDevice));
+  if (!g.stream) HIP_OK(hipStreamCreate(&g.stream));
+  if (!g.ev0) HIP_OK(hipEventCreate(&g.ev0));
+  if (!g.ev1) HIP_OK(hipEventCreate(&g.ev1));
+  this_cuda->strm[0] = g.stream;
+  this_cuda->diffuse_strm[0] = g.stream;
+  g.mesh = reinterpret_cast<unsigned char*>(cuda_mem->mesh_dev);
+  upload_tables(this_run);
+  if (t_err) return;
+  HIP_OK(g.ctr.reserve(1));
+  g.ready = true;
+}
+
+void send_mesh_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                    struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  HIP_OK(hipMemcpyAsync(cuda_mem->mesh_dev, mesh, sizeof(struct fluid_mesh) * g.ncell,
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void recv_mesh_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                    struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  HIP_OK(hipMemcpyAsync(mesh, cuda_mem->mesh_dev, sizeof(struct fluid_mesh) * g.ncell,
+                        hipMemcpyDeviceToHost, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void send_run_param_data(struct run_param* this_run, struct cuda_mem_space* cuda_mem,
+                         struct cuda_param* this_cuda) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!g.ready || !cuda_mem || !this_run) return set_err(ARGOT_ERR_STATE);
+  HIP_OK(hipMemcpy(cuda_mem->this_run_dev, this_run, sizeof(struct run_param),
+                   hipMemcpyHostToDevice));
+  upload_tables(this_run);
+}
+
+void allocate_pinned_segment(struct ray_segment** seg, uint64_t nseg) {
+  t_err = ARGOT_OK;
+  if (hipHostMalloc((void**)seg, sizeof(struct ray_segment) * std::max<uint64_t>(nseg, 1),
+                    hipHostMallocDefault) != hipSuccess) {
+    *seg = nullptr;
+    set_err(ARGOT_ERR_NOMEM);
+  }
+}
+
+void deallocate_pinned_segment(struct ray_segment* seg) {
+  if (seg) (void)hipHostFree(seg);
+}
+
+void allocate_pinned_light_ray_IO(struct light_ray_IO** ray_IO, uint64_t nray) {
+  t_err = ARGOT_OK;
+  if (hipHostMalloc((void**)ray_IO, sizeof(struct light_ray_IO) * std::max<uint64_t>(nray, 1),
+                    hipHostMallocDefault) != hipSuccess) {
+    *ray_IO = nullptr;
+    set_err(ARGOT_ERR_NOMEM);
+  }
+}
+
+void deallocate_pinned_light_ray_IO(struct light_ray_IO* ray_IO) {
+  if (ray_IO) (void)hipHostFree(ray_IO);
+}
+
+void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cuda_mem,
+                             struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  const int64_t nseg = (int64_t)this_run->nseg;
+  if (nseg <= 0) return;
+  if (!seg) return set_err(ARGOT_ERR_ARG);
+  pack_fields();
+  if (t_err) return;
+  const size_t bytes = sizeof(struct ray_segment) * nseg;
+  HIP_OK(g.seg.reserve(bytes));
+  HIP_OK(hipMemcpyAsync(g.seg.p, seg, bytes, hipMemcpyHostToDevice, g.stream));
+  const GridDev gd = make_grid(this_run);
+  HIP_OK(launch_walk_segments(gd, g.nf.p, g.seg.p, nseg, g_tau_mode, g.stream));
+  HIP_OK(hipMemcpyAsync(seg, g.seg.p, bytes, hipMemcpyDeviceToHost, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
+                             struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
+                             struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  const int64_t nray = (int64_t)this_run->nray;
+  if (nray <= 0) return;
+  if (!ray || !ray_IO) return set_err(ARGOT_ERR_ARG);
+  const GridDev gd = make_grid(this_run);
+  // Repack light_ray -> light_ray_IO (calc_photoion_rate_cuda.cu:108-132),
+  // keeping each target's rays together in their original order so the
+  // per-target sums need no atomics.
+  std::vector<int64_t> tgt(nray);
+  for (int64_t i = 0; i < nray; i++) {
+    const int64_t lx = ray[i].ix_target - gd.ox, ly = ray[i].iy_target - gd.oy,
+                  lz = ray[i].iz_target - gd.oz;
+    if (lx < 0 || lx >= g.nx || ly < 0 || ly >= g.ny || lz < 0 || lz >= g.nz)
+      return set_err(ARGOT_ERR_ARG);
+    tgt[i] = lz + (int64_t)g.nz * (ly + (int64_t)g.ny * lx);
+  }
+  bool contiguous = true;
+  {
+    std::vector<unsigned char> seen(g.ncell, 0);
+    for (int64_t i = 0; i < nray && contiguous; i++) {
+      if (i > 0 && tgt[i] == tgt[i - 1]) continue;
+      if (seen[tgt[i]]) contiguous = false;
+      seen[tgt[i]] = 1;
+    }
+  }
+  std::vector<int64_t> order;
+  if (!contiguous) {
+    order.resize(nray);
+    for (int64_t i = 0; i < nray; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return tgt[a] < tgt[b]; });
+  }
+  for (int64_t j = 0; j < nray; j++) {
+    const int64_t i = contiguous ? j : order[j];
+    light_ray_IO& o = ray_IO[j];
+    o.src = ray[i].src;
+    o.ix_target = ray[i].ix_target;
+    o.iy_target = ray[i].iy_target;
+    o.iz_target = ray[i].iz_target;
+    o.optical_depth_HI = ray[i].optical_depth_HI;
+    o.optical_depth_HeI = ray[i].optical_depth_HeI;
+    o.optical_depth_HeII = ray[i].optical_depth_HeII;
+  }
+  const size_t bytes = sizeof(struct light_ray_IO) * nray;
+  HIP_OK(g.rays.reserve(bytes));
+  HIP_OK(g.per_ray.reserve(6 * nray));
+  HIP_OK(g.target.reserve(nray));
+  HIP_OK(hipMemcpyAsync(g.rays.p, ray_IO, bytes, hipMemcpyHostToDevice, g.stream));
+  const RateTables rt = rate_tables();
+  HIP_OK(launch_ray_rates(gd, rt, g.dsig.p, g.mesh, g.rays.p, nray, g.per_ray.p, g.target.p,
+                          g.stream));
+  HIP_OK(launch_merge_ray_rates(g.per_ray.p, g.target.p, nray, g.mesh, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void zero_out_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                                 struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                         struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  std::vector<float> tab(6 * g.ncell);
+  for (int64_t c = 0; c < g.ncell; c++) {
+    const prim_chem& p = mesh[c].prev_chem;
+    tab[6 * c + 0] = p.GammaHI;
+    tab[6 * c + 1] = p.HeatHI;
+    tab[6 * c + 2] = p.GammaHeI;
+    tab[6 * c + 3] = p.GammaHeII;
+    tab[6 * c + 4] = p.HeatHeI;
+    tab[6 * c + 5] = p.HeatHeII;
+  }
+  HIP_OK(g.rate_tab.reserve(6 * g.ncell));
+  HIP_OK(hipMemcpyAsync(g.rate_tab.p, tab.data(), sizeof(float) * 6 * g.ncell,
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(launch_add_rates(g.mesh, g.rate_tab.p, g.ncell, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
+                         struct run_param* this_run, struct mpi_param* this_mpi,
+                         struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                         float dtime) {
+  (void)mesh;
+  (void)this_mpi;
+  (void)this_cuda;
+  (void)dtime;
+  t_err = ARGOT_OK;
+  const double t_start = now_ms();
+  if (!check_ready(cuda_mem, this_run)) return;
+  if (this_run->rank_x || this_run->rank_y || this_run->rank_z ||
+      this_run->xmin_local != this_run->xmin || this_run->ymin_local != this_run->ymin ||
+      this_run->zmin_local != this_run->zmin)
+    return set_err(ARGOT_ERR_ARG);   // the fused step covers one rank's whole box
+  const int64_t nsrc = (int64_t)this_run->nsrc;
+  if (nsrc < 0 || (nsrc > 0 && !src)) return set_err(ARGOT_ERR_ARG);
+  this_run->theta_crit = g_theta;
+
+  // Source tree on the host (construct_tree), uploaded in pre-order.
+  std::vector<float> pos(3 * std::max<int64_t>(nsrc, 1));
+  std::vector<double> spec((size_t)ARGOT_NGRID_NU * std::max<int64_t>(nsrc, 1));
+  for (int64_t s = 0; s < nsrc; s++) {
+    pos[3 * s + 0] = src[s].xpos;
+    pos[3 * s + 1] = src[s].ypos;
+    pos[3 * s + 2] = src[s].zpos;
+    memcpy(&spec[(size_t)s * ARGOT_NGRID_NU], src[s].photon_rate, sizeof(double) * ARGOT_NGRID_NU);
+  }
+  const TreeBox box{this_run->xmin, this_run->ymin, this_run->zmin,
+                    this_run->xmax, this_run->ymax, this_run->zmax};
+  if (build_tree(pos.data(), spec.data(), (int)nsrc, ARGOT_NGRID_NU, box, this_run->theta_crit,
+                 &g.tree) != 0)
+    return set_err(ARGOT_ERR_ARG);
+  const HostTree& T = g.tree;
+  const int nn = T.nnode();
+  HIP_OK(g.link.reserve(nn));
+  HIP_OK(g.cm.reserve(4 * (size_t)nn));
+  HIP_OK(g.spec.reserve((size_t)nn * ARGOT_NGRID_NU));
+  HIP_OK(g.spec64.reserve((size_t)nn * ARGOT_NGRID_NU));
+  HIP_OK(g.spec_scale.reserve(nn));
+  HIP_OK(g.src_pos.reserve(T.src_pos.size()));
+  HIP_OK(hipMemcpyAsync(g.link.p, T.link.data(), sizeof(NodeLink) * nn, hipMemcpyHostToDevice,
+                        g.stream));
+  HIP_OK(hipMemcpyAsync(g.cm.p, T.cm.data(), sizeof(float) * 4 * nn, hipMemcpyHostToDevice,
+                        g.stream));
+  HIP_OK(hipMemcpyAsync(g.spec.p, T.spec.data(), sizeof(float) * T.spec.size(),
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipMemcpyAsync(g.spec64.p, T.spec64.data(), sizeof(double) * T.spec64.size(),
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipMemcpyAsync(g.spec_scale.p, T.spec_scale.data(), sizeof(double) * nn,
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipMemcpyAsync(g.src_pos.p, T.src_pos.data(), sizeof(float) * T.src_pos.size(),
+                        hipMemcpyHostToDevice, g.stream));
+  const double t_built = now_ms();
+
+  // Per-cell FP32 densities, rates zeroed (zero_out_photoion_rate_cuda).
+  pack_fields();
+  if (t_err) return;
+  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
+  HIP_OK(launch_reset_counters(g.ctr.p, g.stream));
+
+  FusedArgs a;
+  a.grid = make_grid(this_run);
+  a.tree.nnode = nsrc > 0 ? nn : 0;
+  a.tree.node_cm = g.cm.p;
+  a.tree.link = g.link.p;
+  a.tree.spec = g.spec.p;
+  a.tree.spec_scale = g.spec_scale.p;
+  a.tree.spec64 = g.spec64.p;
+  a.tree.src_pos = g.src_pos.p;
+  a.rates = rate_tables();
+  a.dsig = g.dsig.p;
+  a.nf = g.nf.p;
+  a.mesh = g.mesh;
+  a.ctr = g.ctr.p;
+  const int capture = g_capture_host != nullptr;
+  if (capture) HIP_OK(g.rec.reserve(std::max<int64_t>(g_capture_cap, 1)));
+  a.rec = capture ? g.rec.p : nullptr;
+  a.rec_cap = capture ? g_capture_cap : 0;
+  const int nblock = std::max(1, g.ncu * g_blocks_per_cu);
+  HIP_OK(hipEventRecord(g.ev0, g.stream));
+  HIP_OK(launch_fused_step(a, g_tau_mode, capture, nblock, g.stream));
+  HIP_OK(hipEventRecord(g.ev1, g.stream));
+  StepCounters hc;
+  HIP_OK(hipMemcpyAsync(&hc, g.ctr.p, sizeof(hc), hipMemcpyDeviceToHost, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+  if (capture) {
+    const int64_t n = std::min<int64_t>((int64_t)hc.nrecord, g_capture_cap);
+    HIP_OK(hipMemcpy(g_capture_host, g.rec.p, sizeof(RayRecord) * n, hipMemcpyDeviceToHost));
+  }
+  g.stats.ntarget = (uint64_t)g.ncell;
+  g.stats.nray = hc.nray;
+  g.stats.ncross = hc.ncross;
+  g.stats.nfallback = hc.nfallback;
+  g.stats.ms_build = t_built - t_start;
+  g.stats.ms_kernel = ms;
+  g.stats.ms_total = now_ms() - t_start;
+  this_run->nray = hc.nray;
+  this_run->nseg = hc.nray;
+}
+
+// ------------------------------------------------------------------------
+// Host-only helpers for tests (no GPU needed).
+
+// sizeof / offsetof of the ABI structs, in a fixed order.
+int argot_hip_abi_layout(int64_t* out, int n) {
+  const int64_t v[] = {
+      (int64_t)sizeof(struct prim_chem),
+      (int64_t)sizeof(struct fluid_mesh),
+      (int64_t)sizeof(struct freq_param),
+      (int64_t)sizeof(struct radiation_src),
+      (int64_t)sizeof(struct cross_section),
+      (int64_t)sizeof(struct run_param),
+      (int64_t)sizeof(struct photoion_rate),
+      (int64_t)sizeof(struct ray_segment),
+      (int64_t)sizeof(struct light_ray),
+      (int64_t)sizeof(struct light_ray_IO),
+      (int64_t)sizeof(struct cuda_param),
+      (int64_t)sizeof(struct cuda_mem_space),
+      (int64_t)offsetof(struct run_param, lunit),
+      (int64_t)offsetof(struct run_param, denstonh),
+      (int64_t)offsetof(struct run_param, freq),
+      (int64_t)offsetof(struct run_param, csect),
+      (int64_t)offsetof(struct run_param, nray),
+      (int64_t)offsetof(struct run_param, nseg),
+      (int64_t)offsetof(struct run_param, xmin_local),
+      (int64_t)offsetof(struct run_param, delta_x),
+      (int64_t)offsetof(struct fluid_mesh, chem),
+      (int64_t)offsetof(struct fluid_mesh, prev_chem),
+      (int64_t)offsetof(struct light_ray_IO, optical_depth_HI),
+      (int64_t)sizeof(RayRecord),
+  };
+  const int m = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < n && i < m; i++) out[i] = v[i];
+  return m;
+}
+
+// Product tree on the host: build from nsrc sources and list the rays of a
+// target point, for comparison with the CPU oracle.
+void* argot_hip_tree_build(const float* pos, const double* spec, int nsrc, int ngrid,
+                           const float* box6, float theta) {
+  HostTree* t = new HostTree();
+  const TreeBox b{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]};
+  if (build_tree(pos, spec, nsrc, ngrid, b, theta, t) != 0) {
+    delete t;
+    return nullptr;
+  }
+  return t;
+}
+
+void argot_hip_tree_free(void* h) { delete static_cast<HostTree*>(h); }
+
+int argot_hip_tree_nnode(void* h) { return static_cast<HostTree*>(h)->nnode(); }
+
+uint64_t argot_hip_tree_rays_to(void* h, const float* target, uint64_t cap, int* kind, int* id,
+                                float* xyz) {
+  std::vector<HostRay> rays;
+  const uint64_t n = tree_rays_to(*static_cast<HostTree*>(h), target, &rays);
+  for (uint64_t i = 0; i < n && i < cap; i++) {
+    kind[i] = rays[i].kind;
+    id[i] = rays[i].id;
+    xyz[3 * i + 0] = rays[i].pos[0];
+    xyz[3 * i + 1] = rays[i].pos[1];
+    xyz[3 * i + 2] = rays[i].pos[2];
+  }
+  return n;
+}
+
+const double* argot_hip_tree_spec(void* h, int node) {
+  HostTree* t = static_cast<HostTree*>(h);
+  return &t->spec64[(size_t)node * t->ngrid];
+}
+
+}  // extern "C"

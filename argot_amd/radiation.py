@@ -1,0 +1,152 @@
+"""Host-side mirror of ARGOT's point-source radiation interface.
+
+Drives libargot_hip.so through its C-ABI the way the reference's host code
+drives its GPU entry points: init_run's table set-up (setup_freq_param,
+setup_cross_section; src/init_run.c:93), init_gpu + send_mesh_data
+(src/argot_mpi.c:68-71), step_radiation_tree (src/step_radiation_tree.c:554)
+and recv_mesh_data.  Inputs are synthetic workloads from argot_amd.ic.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from ._lib import check, lib
+
+RATE_FIELDS = ("GammaHI", "HeatHI", "GammaHeI", "GammaHeII", "HeatHeI", "HeatHeII")
+
+
+def freq_and_csect():
+    """(freq_param, cross_section[256]) from the library's table set-up."""
+    L = lib()
+    freq = abi.FreqParam()
+    L.setup_freq_param(C.byref(freq))
+    cs = (abi.CrossSection * abi.NGRID_NU)()
+    L.setup_cross_section(cs, C.byref(freq))
+    return freq, cs
+
+
+def blackbody_sources(pos, rates, temps, freq=None):
+    """radiation_src[] at pos with blackbody spectra (setup_photon_rate).
+
+    Sources sharing (temperature, total rate) share one spectrum computation;
+    the result is identical to calling setup_photon_rate per source."""
+    L = lib()
+    if freq is None:
+        freq, _ = freq_and_csect()
+    n = len(rates)
+    arr = (abi.RadiationSrc * max(n, 1))()
+    cache = {}
+    for i in range(n):
+        key = (float(np.float32(temps[i])), float(rates[i]))
+        s = arr[i]
+        s.type = 0
+        s.param = float(temps[i])
+        s.xpos, s.ypos, s.zpos = (float(v) for v in pos[i])
+        if key not in cache:
+            L.setup_photon_rate(C.byref(freq), C.byref(s), float(rates[i]))
+            cache[key] = np.ctypeslib.as_array(s.photon_rate).copy()
+        else:
+            np.ctypeslib.as_array(s.photon_rate)[:] = cache[key]
+    return arr
+
+
+def run_param_for(w, freq, cs):
+    """struct run_param of one rank owning the whole box [0,1)^3."""
+    r = abi.RunParam()
+    n = w.n
+    r.helium_flag = 1
+    r.ngrid_nu = abi.NGRID_NU
+    r.mpi_nproc = 1
+    r.nnode_x = r.nnode_y = r.nnode_z = 1
+    r.lunit = w.lunit
+    r.denstonh = w.denstonh
+    r.anow = 1.0
+    r.freq = freq
+    C.memmove(C.addressof(r.csect), C.addressof(cs), C.sizeof(cs))
+    r.nspecies = 5
+    r.nchannel = 3
+    r.nsrc = w.nsrc
+    r.xmin = r.ymin = r.zmin = 0.0
+    r.xmax = r.ymax = r.zmax = 1.0
+    r.xmin_local = r.ymin_local = r.zmin_local = 0.0
+    r.xmax_local = r.ymax_local = r.zmax_local = 1.0
+    r.nmesh_x_total = r.nmesh_y_total = r.nmesh_z_total = n
+    r.nmesh_x_local = r.nmesh_y_local = r.nmesh_z_local = n
+    d = np.float32(np.float32(1.0) / np.float32(n))
+    r.delta_x = r.delta_y = r.delta_z = float(d)
+    r.theta_crit = 0.7
+    r.nmesh_per_loop = n * n * n
+    return r
+
+
+def fill_mesh(w):
+    """struct fluid_mesh[n^3] holding the workload's density and fractions."""
+    ncell = w.n ** 3
+    mesh = abi.mesh_array(ncell)
+    v = abi.mesh_view(mesh)
+    v["dens"] = w.dens.reshape(-1)
+    chem = v["chem"]
+    chem["fHI"] = w.fHI.reshape(-1)
+    chem["fHII"] = w.fHII.reshape(-1)
+    chem["fHeI"] = w.fHeI.reshape(-1)
+    chem["fHeII"] = w.fHeII.reshape(-1)
+    chem["fHeIII"] = w.fHeIII.reshape(-1)
+    v["prev_chem"] = chem
+    return mesh
+
+
+class PointSourceStep:
+    """One GPU, one rank: the reference's GPU-ARGOT call sequence."""
+
+    def __init__(self, w, tau_mode=0, rate_mode=0):
+        L = lib()
+        self.w = w
+        self.freq, self.cs = freq_and_csect()
+        self.run = run_param_for(w, self.freq, self.cs)
+        self.src = blackbody_sources(w.src_pos, w.src_rate, w.src_temp, self.freq)
+        self.mesh = fill_mesh(w)
+        self.cmem = abi.CudaMemSpace()
+        self.cpar = abi.CudaParam()
+        L.argot_hip_set_helium_fact(w.helium_fact)
+        L.argot_hip_set_tau_mode(tau_mode)
+        L.argot_hip_set_rate_mode(rate_mode)
+        L.init_gpu(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("init_gpu")
+        L.send_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("send_mesh_data")
+
+    def step(self):
+        L = lib()
+        L.step_radiation_tree(self.mesh, self.src, C.byref(self.run), None,
+                              C.byref(self.cmem), C.byref(self.cpar), 0.0)
+        check("step_radiation_tree")
+        return self.stats()
+
+    def stats(self):
+        st = abi.StepStats()
+        lib().argot_hip_step_stats(C.byref(st))
+        return {k: getattr(st, k) for k, _ in abi.StepStats._fields_}
+
+    def rates(self):
+        """Per-cell rates [n^3, 6] (GammaHI, HeatHI, GammaHeI, GammaHeII,
+        HeatHeI, HeatHeII) read back from the device mesh."""
+        L = lib()
+        L.recv_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("recv_mesh_data")
+        pc = abi.mesh_view(self.mesh)["prev_chem"]
+        return np.stack([pc[f] for f in RATE_FIELDS], axis=1).astype(np.float32)
+
+    def capture(self, cap):
+        """Run one step recording every ray; returns the records sorted by
+        (target, seq)."""
+        L = lib()
+        rec = np.zeros(cap, dtype=abi.RAY_RECORD)
+        L.argot_hip_set_capture(rec.ctypes.data, cap)
+        try:
+            st = self.step()
+        finally:
+            L.argot_hip_set_capture(None, 0)
+        n = min(int(st["nray"]), cap)
+        rec = rec[:n]
+        return rec[np.lexsort((rec["seq"], rec["target"]))], st

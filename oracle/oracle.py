@@ -1,0 +1,288 @@
+"""ORACLE — test infrastructure, not product code.
+
+ctypes binding of oracle/liboracle.so, the plain-C CPU restatement of ARGOT's
+point-source radiation step.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py load this module; argot_amd never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_lib = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xmin", C.c_float), ("ymin", C.c_float), ("zmin", C.c_float),
+        ("xmax", C.c_float), ("ymax", C.c_float), ("zmax", C.c_float),
+        ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+        ("lunit", C.c_double), ("denstonh", C.c_double),
+        ("helium_fact", C.c_double),
+        ("theta_crit", C.c_float),
+        ("ngrid", C.c_int),
+        ("nu", C.POINTER(C.c_double)),
+        ("cs", C.POINTER(C.c_float)),
+    ]
+
+
+class Fields(C.Structure):
+    _fields_ = [("dens", C.POINTER(C.c_float)), ("fHI", C.POINTER(C.c_float)),
+                ("fHeI", C.POINTER(C.c_float)), ("fHeII", C.POINTER(C.c_float))]
+
+
+class Stats(C.Structure):
+    _fields_ = [("nray", C.c_uint64), ("ncross", C.c_uint64), ("nseg", C.c_uint64)]
+
+
+def lib():
+    """Load liboracle.so (built by `make -C oracle`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle`")
+    L = C.CDLL(LIB_PATH)
+    L.orc_freq_grid.argtypes = [C.c_int, C.c_double, C.c_double, f64p, C.POINTER(C.c_double)]
+    L.orc_photon_rate_bb.argtypes = [C.c_int, C.c_double, C.c_double, C.c_float, C.c_double, f64p]
+    for n in ("orc_csect_HI", "orc_csect_HeI", "orc_csect_HeII"):
+        getattr(L, n).argtypes = [C.c_double]
+        getattr(L, n).restype = C.c_double
+    L.orc_cross_sections.argtypes = [C.c_int, f64p, f32p]
+    L.orc_tree_build.argtypes = [C.POINTER(Grid), C.c_int, f32p, f64p]
+    L.orc_tree_build.restype = C.c_void_p
+    L.orc_tree_free.argtypes = [C.c_void_p]
+    L.orc_tree_nnode.argtypes = [C.c_void_p]
+    L.orc_tree_node.argtypes = [C.c_void_p, C.c_int, f32p, C.POINTER(C.c_float),
+                                C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.POINTER(C.c_double))]
+    L.orc_count_ray_to.argtypes = [C.c_void_p, C.POINTER(Grid), C.c_int, C.c_int, C.c_int]
+    L.orc_count_ray_to.restype = C.c_uint64
+    L.orc_rays_to.argtypes = [C.c_void_p, C.POINTER(Grid), C.c_int, C.c_int, C.c_int,
+                              C.c_uint64, i32p, i32p, f32p]
+    L.orc_rays_to.restype = C.c_uint64
+    L.orc_ray_spec.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.orc_ray_spec.restype = C.POINTER(C.c_double)
+    L.orc_ray_segment.argtypes = [C.POINTER(Grid), f32p, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, f32p, f32p]
+    L.orc_walk_fp32.argtypes = [C.POINTER(Grid), C.POINTER(Fields), f32p, f32p, f32p,
+                                C.c_int, C.c_void_p]
+    L.orc_walk_fp32_kahan.argtypes = [C.POINTER(Grid), C.POINTER(Fields), f32p, f32p, f32p]
+    L.orc_walk_exact.argtypes = [C.POINTER(Grid), C.POINTER(Fields), f32p, f32p, f64p]
+    L.orc_delta_length.argtypes = [C.POINTER(Grid), C.c_float, C.c_float, C.c_float]
+    L.orc_delta_length.restype = C.c_float
+    L.orc_ray_rate.argtypes = [C.POINTER(Grid), C.POINTER(Fields), f32p, f64p, C.c_int,
+                               C.c_int, C.c_int, f64p, C.c_int, f64p]
+    L.orc_step.argtypes = [C.POINTER(Grid), C.POINTER(Fields), C.c_void_p, C.c_int64,
+                           C.c_int64, f32p, C.c_void_p, C.c_int, C.c_int, C.POINTER(Stats)]
+    L.orc_ray_records.argtypes = [C.POINTER(Grid), C.POINTER(Fields), C.c_void_p,
+                                  C.c_int64, C.c_int64, C.c_uint64, i64p, i32p, i32p,
+                                  f32p, i32p, u64p]
+    L.orc_ray_records.restype = C.c_uint64
+    L.orc_thin_kernel.argtypes = [C.c_int, f64p, f32p, f64p, f64p, f64p]
+    _lib = L
+    return L
+
+
+def freq_grid(ngrid=256, log_nu_min=0.0, log_nu_max=1.5):
+    nu = np.zeros(ngrid)
+    d = C.c_double()
+    lib().orc_freq_grid(ngrid, log_nu_min, log_nu_max, nu, C.byref(d))
+    return nu, d.value
+
+
+def photon_rate_bb(temperature=1.0e5, total=1.0, ngrid=256, log_nu_min=0.0, log_nu_max=1.5):
+    out = np.zeros(ngrid)
+    lib().orc_photon_rate_bb(ngrid, log_nu_min, log_nu_max, temperature, total, out)
+    return out
+
+
+def cross_sections(nu):
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    cs = np.zeros(3 * len(nu), dtype=np.float32)
+    lib().orc_cross_sections(len(nu), nu, cs)
+    return cs.reshape(-1, 3)
+
+
+class Problem:
+    """A mesh + sources in the oracle's terms (box [0,1)^3 by default)."""
+
+    def __init__(self, dens, fHI, fHeI, fHeII, src_pos, src_spec, *, lunit, denstonh,
+                 helium_fact, theta=0.7, nu=None, cs=None, box=(0.0, 1.0)):
+        L = lib()
+        self.shape = dens.shape
+        nx, ny, nz = dens.shape
+        self.dens = np.ascontiguousarray(dens, dtype=np.float32)
+        self.fHI = np.ascontiguousarray(fHI, dtype=np.float32)
+        self.fHeI = np.ascontiguousarray(fHeI, dtype=np.float32)
+        self.fHeII = np.ascontiguousarray(fHeII, dtype=np.float32)
+        if nu is None:
+            nu, _ = freq_grid()
+        self.nu = np.ascontiguousarray(nu, dtype=np.float64)
+        self.cs = np.ascontiguousarray(cross_sections(self.nu) if cs is None else cs,
+                                       dtype=np.float32).reshape(-1)
+        lo, hi = box
+        g = Grid()
+        g.nx, g.ny, g.nz = nx, ny, nz
+        g.xmin = g.ymin = g.zmin = lo
+        g.xmax = g.ymax = g.zmax = hi
+        g.dx = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(nx))
+        g.dy = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(ny))
+        g.dz = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(nz))
+        g.lunit, g.denstonh, g.helium_fact = lunit, denstonh, helium_fact
+        g.theta_crit = theta
+        g.ngrid = len(self.nu)
+        g.nu = self.nu.ctypes.data_as(C.POINTER(C.c_double))
+        g.cs = self.cs.ctypes.data_as(C.POINTER(C.c_float))
+        self.grid = g
+        f = Fields()
+        f.dens = self.dens.ctypes.data_as(C.POINTER(C.c_float))
+        f.fHI = self.fHI.ctypes.data_as(C.POINTER(C.c_float))
+        f.fHeI = self.fHeI.ctypes.data_as(C.POINTER(C.c_float))
+        f.fHeII = self.fHeII.ctypes.data_as(C.POINTER(C.c_float))
+        self.fields = f
+        self.src_pos = np.ascontiguousarray(src_pos, dtype=np.float32).reshape(-1)
+        self.src_spec = np.ascontiguousarray(src_spec, dtype=np.float64).reshape(-1)
+        self.nsrc = len(self.src_pos) // 3
+        self.tree = L.orc_tree_build(C.byref(g), self.nsrc, self.src_pos, self.src_spec)
+
+    def __del__(self):
+        try:
+            if self.tree:
+                lib().orc_tree_free(self.tree)
+        except Exception:
+            pass
+
+    @property
+    def ncell(self):
+        return int(np.prod(self.shape))
+
+    def count_ray_to(self, ix, iy, iz):
+        return int(lib().orc_count_ray_to(self.tree, C.byref(self.grid), ix, iy, iz))
+
+    def rays_to(self, ix, iy, iz):
+        L = lib()
+        n = self.count_ray_to(ix, iy, iz)
+        kind = np.zeros(max(n, 1), np.int32)
+        ids = np.zeros(max(n, 1), np.int32)
+        pos = np.zeros(3 * max(n, 1), np.float32)
+        L.orc_rays_to(self.tree, C.byref(self.grid), ix, iy, iz, n, kind, ids, pos)
+        return kind[:n], ids[:n], pos[:3 * n].reshape(-1, 3)
+
+    def ray_spec(self, kind, ident):
+        p = lib().orc_ray_spec(self.tree, int(kind), int(ident))
+        return np.ctypeslib.as_array(p, shape=(self.grid.ngrid,)).copy()
+
+    def node(self, k):
+        cm = np.zeros(3, np.float32)
+        length, n, nch = C.c_float(), C.c_int(), C.c_int()
+        sp = C.POINTER(C.c_double)()
+        lib().orc_tree_node(self.tree, k, cm, C.byref(length), C.byref(n), C.byref(nch),
+                            C.byref(sp))
+        spec = np.ctypeslib.as_array(sp, shape=(self.grid.ngrid,)).copy()
+        return cm, length.value, n.value, nch.value, spec
+
+    def nnode(self):
+        return int(lib().orc_tree_nnode(self.tree))
+
+    def ray_segment(self, src, ix, iy, iz, maxseg=8):
+        s = np.zeros(3 * maxseg, np.float32)
+        e = np.zeros(3 * maxseg, np.float32)
+        n = lib().orc_ray_segment(C.byref(self.grid), np.ascontiguousarray(src, np.float32),
+                                  ix, iy, iz, maxseg, s, e)
+        return s[:3 * n].reshape(-1, 3), e[:3 * n].reshape(-1, 3)
+
+    def walk(self, start, end, cells=False):
+        tau = np.zeros(3, np.float32)
+        cap = sum(self.shape) + 8
+        buf = np.zeros(cap, np.int32)
+        n = lib().orc_walk_fp32(C.byref(self.grid), C.byref(self.fields),
+                                np.ascontiguousarray(start, np.float32),
+                                np.ascontiguousarray(end, np.float32), tau, cap,
+                                buf.ctypes.data_as(C.c_void_p))
+        if cells:
+            return tau, n, buf[:min(n, cap)].copy()
+        return tau, n
+
+    def walk_kahan(self, start, end):
+        tau = np.zeros(3, np.float32)
+        n = lib().orc_walk_fp32_kahan(C.byref(self.grid), C.byref(self.fields),
+                                      np.ascontiguousarray(start, np.float32),
+                                      np.ascontiguousarray(end, np.float32), tau)
+        return tau, n
+
+    def walk_exact(self, start, end):
+        tau = np.zeros(3, np.float64)
+        n = lib().orc_walk_exact(C.byref(self.grid), C.byref(self.fields),
+                                 np.ascontiguousarray(start, np.float32),
+                                 np.ascontiguousarray(end, np.float32), tau)
+        return tau, n
+
+    def delta_length(self, dx, dy, dz):
+        return lib().orc_delta_length(C.byref(self.grid), dx, dy, dz)
+
+    def ray_rate(self, src, spec, ix, iy, iz, tau, mode=0):
+        out = np.zeros(6)
+        lib().orc_ray_rate(C.byref(self.grid), C.byref(self.fields),
+                           np.ascontiguousarray(src, np.float32),
+                           np.ascontiguousarray(spec, np.float64), ix, iy, iz,
+                           np.ascontiguousarray(tau, np.float64), mode, out)
+        return out
+
+    def step(self, t0=0, t1=None, f64=False, kahan=False, nthreads=0):
+        """Rates for targets [t0, t1): (rates_ref float32 [N,6], rates_f64 or None, stats)."""
+        if t1 is None:
+            t1 = self.ncell
+        r32 = np.zeros(6 * self.ncell, np.float32)
+        r64 = np.zeros(6 * self.ncell, np.float64) if f64 else None
+        st = Stats()
+        lib().orc_step(C.byref(self.grid), C.byref(self.fields), self.tree, t0, t1, r32,
+                       r64.ctypes.data_as(C.c_void_p) if f64 else None, int(kahan),
+                       int(nthreads), C.byref(st))
+        return (r32.reshape(-1, 6), None if r64 is None else r64.reshape(-1, 6),
+                {"nray": st.nray, "ncross": st.ncross, "nseg": st.nseg})
+
+    def ray_records(self, t0, t1):
+        """Per-ray (target, kind, id, tau[3], ncross, cell-sequence hash)."""
+        L = lib()
+        n = sum(self.count_ray_to(*cell_index(im, self.shape)) for im in range(t0, t1))
+        m = max(n, 1)
+        tgt = np.zeros(m, np.int64)
+        kind = np.zeros(m, np.int32)
+        ids = np.zeros(m, np.int32)
+        tau = np.zeros(3 * m, np.float32)
+        nc = np.zeros(m, np.int32)
+        h = np.zeros(m, np.uint64)
+        w = L.orc_ray_records(C.byref(self.grid), C.byref(self.fields), self.tree, t0, t1,
+                              n, tgt, kind, ids, tau, nc, h)
+        assert w == n
+        return {"target": tgt[:n], "kind": kind[:n], "id": ids[:n],
+                "tau": tau[:3 * n].reshape(-1, 3), "ncross": nc[:n], "hash": h[:n]}
+
+
+def cell_index(im, shape):
+    nx, ny, nz = shape
+    ix = im // (ny * nz)
+    iy = (im - ix * ny * nz) // nz
+    iz = im - ix * ny * nz - iy * nz
+    return int(ix), int(iy), int(iz)
+
+
+def thin_kernel(nu, cs, spec, tau_th):
+    out = np.zeros(4)
+    lib().orc_thin_kernel(len(nu), np.ascontiguousarray(nu, np.float64),
+                          np.ascontiguousarray(cs, np.float32).reshape(-1),
+                          np.ascontiguousarray(spec, np.float64),
+                          np.ascontiguousarray(tau_th, np.float64), out)
+    return out

@@ -1,0 +1,65 @@
+"""Shared set-up for the parity tests: one workload, given to both the HIP
+library (through its C-ABI) and the CPU oracle, with identical inputs (the
+run_param tables and the radiation_src spectra are inputs of the step)."""
+import numpy as np
+
+import oracle as O
+from argot_amd import ic
+from argot_amd import radiation as R
+
+
+def spectra_of(srcs, n):
+    return np.array([np.ctypeslib.as_array(srcs[i].photon_rate) for i in range(n)])
+
+
+def tables():
+    freq, cs = R.freq_and_csect()
+    nu = np.ctypeslib.as_array(freq.nu).copy()
+    csa = np.array([[c.csect_HI, c.csect_HeI, c.csect_HeII] for c in cs], dtype=np.float32)
+    return freq, cs, nu, csa
+
+
+def oracle_problem(w, theta=0.7):
+    freq, cs, nu, csa = tables()
+    srcs = R.blackbody_sources(w.src_pos, w.src_rate, w.src_temp, freq)
+    spec = spectra_of(srcs, w.nsrc)
+    return O.Problem(w.dens, w.fHI, w.fHeI, w.fHeII, w.src_pos, spec, lunit=w.lunit,
+                     denstonh=w.denstonh, helium_fact=w.helium_fact, theta=theta, nu=nu,
+                     cs=csa)
+
+
+def small_workload(n=16, nsrc=5, seed=7, lognormal=True, inside_cells=True):
+    """A tiny mesh for exhaustive checks (every target, every ray)."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    if lognormal:
+        dens = ic.lognormal_density(n, seed + 99)
+        fHI = rng.uniform(1e-4, 1.0, shape).astype(np.float32)
+        fHeI = rng.uniform(1e-4, 1.0, shape).astype(np.float32)
+    else:
+        dens = np.ones(shape, np.float32)
+        fHI = np.full(shape, 0.999, np.float32)
+        fHeI = np.full(shape, 0.999, np.float32)
+    fHII = (1 - fHI).astype(np.float32)
+    fHeII = (1 - fHeI).astype(np.float32)
+    fHeIII = np.zeros(shape, np.float32)
+    if inside_cells:
+        pos = ((rng.integers(0, n, (nsrc, 3)) + 0.5) / n).astype(np.float32)
+    else:
+        pos = rng.uniform(0.0, 1.0, (nsrc, 3)).astype(np.float32)
+    rates = ic.power_law_rates(rng, nsrc)
+    return ic.Workload(f"small{n}", n, dens, fHI, fHII, fHeI, fHeII, fHeIII, pos, rates,
+                       np.full(nsrc, 1.0e5, np.float32))
+
+
+def rel_err(a, b, floor):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def cell_centre(idx, n, lo=0.0, hi=1.0):
+    """Cell centre as the reference forms it: lo + ((float)i + 0.5)*delta in
+    double, rounded to float (setup_light_ray_tree.c:341-343)."""
+    d = np.float64(np.float32(np.float32(hi - lo) / np.float32(n)))
+    return np.float32(np.float64(np.float32(lo)) + (np.float64(np.float32(idx)) + 0.5) * d)

@@ -1,0 +1,128 @@
+"""The reference's per-call GPU entry points through the C-ABI:
+calc_optical_depth_cuda on host segment lists and calc_photoion_rate_cuda on
+light_ray lists, against the CPU oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from argot_amd import abi
+from argot_amd import radiation as R
+from argot_amd._lib import check, lib
+from helpers import oracle_problem, rel_err, small_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def random_segments(rng, n, count, inside=True):
+    if inside:
+        s = rng.uniform(0.0, 1.0, (count, 3)).astype(np.float32)
+        e = rng.uniform(0.0, 1.0, (count, 3)).astype(np.float32)
+    else:
+        s = rng.uniform(-0.2, 1.2, (count, 3)).astype(np.float32)
+        e = rng.uniform(-0.2, 1.2, (count, 3)).astype(np.float32)
+    # a share of lattice-aligned segments: centre to centre (exact ties),
+    # axis-parallel, and zero-length
+    k = count // 4
+    a = (rng.integers(0, n, (k, 3)) + 0.5) / n
+    b = (rng.integers(0, n, (k, 3)) + 0.5) / n
+    s[:k] = a
+    e[:k] = b
+    e[k:k + 8] = s[k:k + 8]
+    e[k + 8:k + 40, 1:] = s[k + 8:k + 40, 1:]
+    return s, e
+
+
+@pytest.mark.parametrize("kahan", [0, 1])
+def test_calc_optical_depth_cuda(require_gpu, kahan):
+    w = small_workload(n=48, nsrc=3, seed=21)
+    P = oracle_problem(w)
+    step = R.PointSourceStep(w)
+    L = lib()
+    rng = np.random.default_rng(5)
+    s, e = random_segments(rng, w.n, 4000)
+    segp = C.c_void_p()
+    L.allocate_pinned_segment(C.byref(segp), len(s))
+    check("allocate_pinned_segment")
+    try:
+        seg = (abi.RaySegment * len(s)).from_address(segp.value)
+        for i in range(len(s)):
+            seg[i].ray_indx = i
+            seg[i].xpos_start, seg[i].ypos_start, seg[i].zpos_start = (float(v) for v in s[i])
+            seg[i].xpos_end, seg[i].ypos_end, seg[i].zpos_end = (float(v) for v in e[i])
+        step.run.nseg = len(s)
+        L.argot_hip_set_tau_mode(kahan)
+        L.calc_optical_depth_cuda(segp, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+        check("calc_optical_depth_cuda")
+        got = np.array([[seg[i].optical_depth_HI, seg[i].optical_depth_HeI,
+                         seg[i].optical_depth_HeII] for i in range(len(s))], np.float32)
+    finally:
+        L.argot_hip_set_tau_mode(0)
+        L.deallocate_pinned_segment(segp)
+    for i in range(len(s)):
+        tau = P.walk_kahan(s[i], e[i])[0] if kahan else P.walk(s[i], e[i])[0]
+        assert np.array_equal(tau.view(np.uint32), got[i].view(np.uint32)), (i, s[i], e[i])
+
+
+def test_calc_photoion_rate_cuda(require_gpu):
+    w = small_workload(n=16, nsrc=5, seed=8)
+    P = oracle_problem(w)
+    step = R.PointSourceStep(w)
+    L = lib()
+    n = w.n
+    # every ray of every target, with the oracle's optical depths
+    rays = []
+    for im in range(n ** 3):
+        ix, iy, iz = im // (n * n), (im // n) % n, im % n
+        kind, ids, pos = P.rays_to(ix, iy, iz)
+        for k, j, p in zip(kind, ids, pos):
+            s, e = P.ray_segment(p, ix, iy, iz)
+            tau, _ = P.walk(s[-1], e[-1])
+            rays.append((im, ix, iy, iz, P.ray_spec(k, j), p, tau))
+    nray = len(rays)
+    arr = (abi.LightRay * nray)()
+    for i, (im, ix, iy, iz, spec, p, tau) in enumerate(rays):
+        r = arr[i]
+        r.src.type = 0
+        r.src.param = 1.0e5
+        r.src.xpos, r.src.ypos, r.src.zpos = (float(v) for v in p)
+        np.ctypeslib.as_array(r.src.photon_rate)[:] = spec
+        r.ix_target, r.iy_target, r.iz_target = ix, iy, iz
+        r.optical_depth_HI, r.optical_depth_HeI, r.optical_depth_HeII = (float(v) for v in tau)
+    iop = C.c_void_p()
+    L.allocate_pinned_light_ray_IO(C.byref(iop), nray)
+    check("allocate_pinned_light_ray_IO")
+    try:
+        L.zero_out_photoion_rate_cuda(C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+        check("zero_out_photoion_rate_cuda")
+        step.run.nray = nray
+        L.calc_photoion_rate_cuda(C.byref(step.cmem), arr, iop, C.byref(step.cpar),
+                                  C.byref(step.run))
+        check("calc_photoion_rate_cuda")
+    finally:
+        L.deallocate_pinned_light_ray_IO(iop)
+    gpu = step.rates()
+    r32, _, _ = P.step()
+    fin = np.isfinite(r32)
+    assert np.array_equal(fin, np.isfinite(gpu))
+    err = rel_err(gpu[fin], r32[fin], 1e-30)
+    assert err.max() < 1e-5, err.max()
+
+
+def test_merge_photoion_data(require_gpu):
+    w = small_workload(n=8, nsrc=2, seed=1)
+    step = R.PointSourceStep(w)
+    L = lib()
+    L.zero_out_photoion_rate_cuda(C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+    check("zero_out_photoion_rate_cuda")
+    v = abi.mesh_view(step.mesh)
+    rng = np.random.default_rng(0)
+    add = rng.random((w.n ** 3, 6)).astype(np.float32)
+    for j, f in enumerate(R.RATE_FIELDS):
+        v["prev_chem"][f] = add[:, j]
+    L.merge_photoion_data(step.mesh, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+    check("merge_photoion_data")
+    L.merge_photoion_data(step.mesh, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+    check("merge_photoion_data")
+    got = step.rates()
+    assert np.array_equal(got, add + add)
