@@ -18,6 +18,14 @@ constexpr double kRateTiny = 1.0e-31;   // calc_photoion_rate.c:16
 constexpr double kPi = 3.1415926535897932384626433;  // constants.h:24
 constexpr double kHnuL = 6.626e-27 * 3.28e15;          // hplanck*nuL
 
+// Load through the constant address space: with a wave-uniform address the
+// compiler emits a scalar load (s_load), keeping tables out of VGPRs and off
+// the vector memory path.  Only for memory no kernel writes while it runs.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)(p);
+}
+
 // Quotient a/b rounded to float exactly as IEEE float division does, from
 // the double reciprocal rb = 1.0/(double)b (rounded to double).  The
 // double product is within 2^-52 (relative) of a/b, while a quotient of two
@@ -120,11 +128,12 @@ __device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3
 struct Dda {
   int ix, iy, iz;        // current cell
   int ixe, iye, ize;     // end cell (clamped)
-  int64_t idx;           // linear index of the current cell
+  uint32_t idx;          // linear index of the current cell
   float xo, yo, zo;      // direction cosines (with the TINY guards)
   double rxo, ryo, rzo;  // 1/xo etc. in double (exact-division helper)
   float xsf, ysf, zsf;   // x_step etc. as float (0 or 1)
   int xn, yn, zn;        // +-1
+  uint32_t sx, sy, sz;   // linear-index step for each axis (xn*ny*nz, yn*nz, zn)
   float xc, yc, zc;      // position in local coordinates
 };
 
@@ -141,7 +150,7 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->ix = max(min(ix, g.nx - 1), 0);
   d->iy = max(min(iy, g.ny - 1), 0);
   d->iz = max(min(iz, g.nz - 1), 0);
-  d->idx = (int64_t)d->iz + (int64_t)g.nz * ((int64_t)d->iy + (int64_t)g.ny * d->ix);
+  d->idx = (uint32_t)d->iz + (uint32_t)g.nz * ((uint32_t)d->iy + (uint32_t)g.ny * d->ix);
 
   float dx = e.x - s.x, dy = e.y - s.y, dz = e.z - s.z;
   float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kWalkTiny);
@@ -169,14 +178,19 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->xn = px ? 1 : -1;
   d->yn = py ? 1 : -1;
   d->zn = pz ? 1 : -1;
+  d->sx = (uint32_t)(d->xn * (int)g.nyz);
+  d->sy = (uint32_t)(d->yn * g.nz);
+  d->sz = (uint32_t)d->zn;
   d->xc = s.x - g.xmin;
   d->yc = s.y - g.ymin;
   d->zc = s.z - g.zmin;
 }
 
 __device__ __forceinline__ bool dda_active(const GridDev& g, const Dda& d) {
-  return (d.ix != d.ixe || d.iy != d.iye || d.iz != d.ize) && (0 <= d.ix && d.ix < g.nx) &&
-         (0 <= d.iy && d.iy < g.ny) && (0 <= d.iz && d.iz < g.nz);
+  const bool moving = (d.ix != d.ixe) | (d.iy != d.iye) | (d.iz != d.ize);
+  const bool inside = ((unsigned)d.ix < (unsigned)g.nx) & ((unsigned)d.iy < (unsigned)g.ny) &
+                      ((unsigned)d.iz < (unsigned)g.nz);
+  return moving & inside;
 }
 
 // One step of calc_optical_depth.c:163-179; returns the chord length.
@@ -188,18 +202,11 @@ __device__ __forceinline__ float dda_step(const GridDev& g, Dda* d) {
   const float ry = div_by(ddy, d->ryo);
   const float rz = div_by(ddz, d->rzo);
   const float rmin = fminf(rx, fminf(ry, rz));
-  if (rmin == rx) {
-    d->ix += d->xn;
-    d->idx += (int64_t)d->xn * g.nyz;
-  }
-  if (rmin == ry) {
-    d->iy += d->yn;
-    d->idx += (int64_t)d->yn * g.nz;
-  }
-  if (rmin == rz) {
-    d->iz += d->zn;
-    d->idx += d->zn;
-  }
+  const bool ax = rmin == rx, ay = rmin == ry, az = rmin == rz;
+  d->ix += ax ? d->xn : 0;
+  d->iy += ay ? d->yn : 0;
+  d->iz += az ? d->zn : 0;
+  d->idx += (ax ? d->sx : 0u) + (ay ? d->sy : 0u) + (az ? d->sz : 0u);
   d->xc = d->xc + rmin * d->xo;
   d->yc = d->yc + rmin * d->yo;
   d->zc = d->zc + rmin * d->zo;
@@ -228,8 +235,10 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float term) {
 
 // The whole walk of one segment (calc_optical_depth.c:76-227).  nf holds the
 // per-cell FP32 densities (nHI, nHeI, nHeII) formed as the reference forms
-// them.  KAHAN selects compensated sums; HASH records the cell sequence
-// (FNV-1a over the linear cell indices, the end cell last).
+// them.  The next cell's densities are fetched before the current chord is
+// summed, so one gather is always in flight.  KAHAN selects compensated
+// sums; HASH records the cell sequence (FNV-1a over the linear cell
+// indices, the end cell last).
 template <bool KAHAN, bool HASH>
 __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__ nf, Vec3f s,
                                     Vec3f e, WalkOut* out) {
@@ -239,11 +248,14 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
-  while (dda_active(g, d)) {
-    const float4 n = nf[d.idx];
-    if (HASH) h = (h ^ (uint64_t)(uint32_t)d.idx) * 0x100000001b3ull;
+  bool act = dda_active(g, d);
+  float4 n = nf[act ? d.idx : 0u];
+  while (act) {
+    if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
     nc++;
     const float rmin = dda_step(g, &d);
+    const bool nact = dda_active(g, d);
+    const float4 nn = nf[nact ? d.idx : 0u];
     if (KAHAN) {
       kahan_add(t0, c0, rmin * n.x);
       kahan_add(t1, c1, rmin * n.y);
@@ -253,11 +265,13 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
       t1 = t1 + rmin * n.y;
       t2 = t2 + rmin * n.z;
     }
+    n = nn;
+    act = nact;
   }
   const float depth = dda_depth(g, d, e);
-  const int64_t ie = (int64_t)d.ize + (int64_t)g.nz * ((int64_t)d.iye + (int64_t)g.ny * d.ixe);
-  const float4 n = nf[ie];
-  if (HASH) h = (h ^ (uint64_t)(uint32_t)ie) * 0x100000001b3ull;
+  const uint32_t ie = (uint32_t)d.ize + (uint32_t)g.nz * ((uint32_t)d.iye + (uint32_t)g.ny * d.ixe);
+  n = nf[ie];
+  if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
   nc++;
   if (KAHAN) {
     kahan_add(t0, c0, depth * n.x);
@@ -370,11 +384,16 @@ __device__ inline void rate_loop_f64(const RateTables& rt, const RayRate& rr, Sp
   out[5] = h2 * kHnuL;
 }
 
-// (1 - exp(-x))/x in FP32: series below 0.25 (8 terms, truncation < 1e-9),
-// the direct form above.
+// (1 - exp(-x))/x in FP32: Taylor series sum_n (-x)^n/(n+1)! through x^6
+// below 0.25 (truncation < 2e-9), the direct form above (rounding of
+// 1 - exp(-x) costs at most 3e-7 relative there).
 __device__ __forceinline__ float one_minus_exp_over(float x) {
-  const float p = 1.0f + x * (-1.0f / 2 + x * (1.0f / 6 + x * (-1.0f / 24 + x * (1.0f / 120 +
-                  x * (-1.0f / 720 + x * (1.0f / 5040 + x * (-1.0f / 40320)))))));
+  float p = __builtin_fmaf(x, 1.0f / 5040, -1.0f / 720);
+  p = __builtin_fmaf(x, p, 1.0f / 120);
+  p = __builtin_fmaf(x, p, -1.0f / 24);
+  p = __builtin_fmaf(x, p, 1.0f / 6);
+  p = __builtin_fmaf(x, p, -1.0f / 2);
+  p = __builtin_fmaf(x, p, 1.0f);
   const float q = (1.0f - __expf(-x)) * __builtin_amdgcn_rcpf(x);
   return x < 0.25f ? p : q;
 }

@@ -37,10 +37,13 @@ struct GridDev {
 // nu_k > nu_s else 0, with nu_k the FP32 grid value as the reference uses.
 struct RateTables {
   int ngrid;
+  int k_he1;           // bins below k_he1 have sigma_HeI = sigma_HeII = 0
+  int k_he2;           // bins below k_he2 have sigma_HeII = 0
   const float* nu;     // [ngrid]
   const float* cs;     // [3*ngrid], bin-major as run_param.csect
   const float* sig;    // [3*ngrid], species-major
   const float* hsig;   // [3*ngrid], species-major
+  const float* dsig;   // [3*ngrid] sigma_s(k) - sigma_s(last bin), species-major
   double cs_max[3];    // max over bins of sigma_s
   double cs_last[3];   // sigma_s at the last bin
 };

@@ -10,7 +10,6 @@ struct FusedArgs {
   GridDev grid;
   TreeDev tree;
   RateTables rates;
-  const float* dsig;           // [3*ngrid] sigma_s(k) - sigma_s(last), species-major
   const float4* nf;            // per-cell (nHI, nHeI, nHeII, 0)
   unsigned char* mesh;         // AoS fluid_mesh (reads fractions, adds rates)
   StepCounters* ctr;
@@ -24,8 +23,7 @@ hipError_t launch_zero_rates(void* mesh, int64_t ncell, hipStream_t s);
 hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s);
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
                                 int kahan, hipStream_t s);
-hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const float* dsig,
-                            const void* mesh, const void* rays, int64_t nray, double* per_ray,
+hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* mesh, const void* rays, int64_t nray, double* per_ray,
                             int64_t* target, hipStream_t s);
 hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
                                   void* mesh, hipStream_t s);

@@ -80,49 +80,74 @@ __global__ void walk_segments_kernel(GridDev g, const float4* __restrict__ nf,
   rec[12] = w.tau[2];
 }
 
+// Bins [k0, k1) of the FP32 frequency loop with NS active species (the
+// species s >= NS have sigma_s = 0 in these bins, so their optical depth
+// enters only through the constant cst).  Tables are read with scalar loads
+// (wave-uniform bin index); sums run in FP32 over 32 bins, then in FP64.
+template <int NS, typename SpecF>
+__device__ __forceinline__ void rate_bins(const RateTables& rt, int k0, int k1, float cst,
+                                          const float n[3], const float d[3], SpecF spec,
+                                          double G[3], double H[3]) {
+  const int ng = rt.ngrid;
+  for (int c = k0; c < k1; c += 32) {
+    float g[NS], h[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) g[s] = h[s] = 0.0f;
+    const int ce = min(c + 32, k1);
+#pragma unroll 4
+    for (int k = c; k < ce; k++) {
+      float sg[NS], hs[NS];
+      float tau = cst, dt = 0.0f;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        sg[s] = ldc(rt.sig + s * ng + k);
+        hs[s] = ldc(rt.hsig + s * ng + k);
+        tau = __builtin_fmaf(n[s], ldc(rt.dsig + s * ng + k), tau);
+        dt = __builtin_fmaf(d[s], sg[s], dt);
+      }
+      const float b = spec(k) * __expf(-tau) * one_minus_exp_over(dt);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        g[s] = __builtin_fmaf(sg[s], b, g[s]);
+        h[s] = __builtin_fmaf(hs[s], b, h[s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      G[s] += g[s];
+      H[s] += h[s];
+    }
+  }
+}
+
 // FP32 frequency loop (photon-conserving form, calc_photoion_rate.c:327-386
 // regrouped): with B_k = N_k exp(-tau_k) (1 - exp(-dtau_k)) / dtau_k,
 //   Gamma_s = r_s d_s sum_k sigma_s(k) B_k,  Heat_s = r_s d_s sum_k hsig_s(k) B_k.
 // The optical depth is shifted by its last-bin value tsh (folded back in
-// FP64), sums run in FP32 over 32 bins and in FP64 across chunks.
+// FP64): tau_k - tsh = sum_s N_s (sigma_s(k) - sigma_s(last)).  Bins below
+// the HeI and HeII edges skip the absent species.
 template <typename SpecF>
-__device__ inline void rate_loop_f32(const RateTables& rt, const float* __restrict__ dsig,
-                                     const RayRate& rr, double scale, SpecF spec,
-                                     double out[6]) {
-  const int ng = rt.ngrid;
+__device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, double scale,
+                                     SpecF spec, double out[6]) {
   const double tsh = rr.t[0] * rt.cs_last[0] + rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2];
-  const float n0 = (float)rr.t[0], n1 = (float)rr.t[1], n2 = (float)rr.t[2];
-  const float d0 = (float)rr.d[0], d1 = (float)rr.d[1], d2 = (float)rr.d[2];
-  double G0 = 0, G1 = 0, G2 = 0, H0 = 0, H1 = 0, H2 = 0;
-  for (int c = 0; c < ng; c += 32) {
-    float g0 = 0, g1 = 0, g2 = 0, h0 = 0, h1 = 0, h2 = 0;
-    const int ce = min(c + 32, ng);
-    for (int k = c; k < ce; k++) {
-      const float s0 = rt.sig[k], s1 = rt.sig[ng + k], s2 = rt.sig[2 * ng + k];
-      const float tau = __builtin_fmaf(n2, dsig[2 * ng + k],
-                                       __builtin_fmaf(n1, dsig[ng + k], n0 * dsig[k]));
-      const float dt = __builtin_fmaf(d2, s2, __builtin_fmaf(d1, s1, d0 * s0));
-      const float b = spec(k) * __expf(-tau) * one_minus_exp_over(dt);
-      g0 = __builtin_fmaf(s0, b, g0);
-      g1 = __builtin_fmaf(s1, b, g1);
-      g2 = __builtin_fmaf(s2, b, g2);
-      h0 = __builtin_fmaf(rt.hsig[k], b, h0);
-      h1 = __builtin_fmaf(rt.hsig[ng + k], b, h1);
-      h2 = __builtin_fmaf(rt.hsig[2 * ng + k], b, h2);
-    }
-    G0 += g0; G1 += g1; G2 += g2;
-    H0 += h0; H1 += h1; H2 += h2;
-  }
+  const float n[3] = {(float)rr.t[0], (float)rr.t[1], (float)rr.t[2]};
+  const float d[3] = {(float)rr.d[0], (float)rr.d[1], (float)rr.d[2]};
+  const float c1 = (float)(-(rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2]));
+  const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
+  double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+  rate_bins<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
+  rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
+  rate_bins<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
   const double pre = scale * exp(-tsh);
   const double p0 = rr.r[0] * rr.d[0] * pre;
   const double p1 = rr.r[1] * rr.d[1] * pre;
   const double p2 = rr.r[2] * rr.d[2] * pre;
-  out[0] = p0 * G0;
-  out[1] = p0 * H0 * kHnuL;
-  out[2] = p1 * G1;
-  out[3] = p2 * G2;
-  out[4] = p1 * H1 * kHnuL;
-  out[5] = p2 * H2 * kHnuL;
+  out[0] = p0 * G[0];
+  out[1] = p0 * H[0] * kHnuL;
+  out[2] = p1 * G[1];
+  out[3] = p2 * G[2];
+  out[4] = p1 * H[1] * kHnuL;
+  out[5] = p2 * H[2] * kHnuL;
 }
 
 // Lower bound of the shifted bin optical depth and upper bound of the
@@ -141,8 +166,7 @@ __device__ __forceinline__ bool fast_rate_ok(const RateTables& rt, const RayRate
 // Legacy per-ray rate kernel: one thread per light_ray_IO record (2096 B:
 // src@0 {type, param, xpos@8, ypos, zpos, photon_rate@24}, ix_target@2072,
 // tau@2084).  Per-ray results go to out[6*i] for the segmented merge.
-__global__ void ray_rates_kernel(GridDev g, RateTables rt, const float* __restrict__ dsig,
-                                 const unsigned char* __restrict__ mesh,
+__global__ void ray_rates_kernel(GridDev g, RateTables rt, const unsigned char* __restrict__ mesh,
                                  const unsigned char* __restrict__ rays, int64_t nray,
                                  double* __restrict__ out, int64_t* __restrict__ target) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,7 +193,7 @@ __global__ void ray_rates_kernel(GridDev g, RateTables rt, const float* __restri
     double smax = 0.0;
     for (int k = 0; k < rt.ngrid; k++) smax = fmax(smax, spec[k]);
     const double inv = smax > 0.0 ? 1.0 / smax : 0.0;
-    rate_loop_f32(rt, dsig, rr, smax, [&](int k) { return (float)(spec[k] * inv); }, res);
+    rate_loop_f32(rt, rr, smax, [&](int k) { return (float)(spec[k] * inv); }, res);
   } else {
     rate_loop_f64(rt, rr, [&](int k) { return spec[k]; }, res);
   }
@@ -222,8 +246,16 @@ __device__ __forceinline__ int64_t next_tile(StepCounters* ctr, int64_t ntile, i
   return -1;
 }
 
+// ARGOT_WAVES_PER_EU (build option) caps the register budget so that this
+// many wavefronts fit per SIMD.
+#if defined(ARGOT_WAVES_PER_EU) && ARGOT_WAVES_PER_EU > 0
+#define ARGOT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARGOT_WAVES_PER_EU)))
+#else
+#define ARGOT_OCCUPANCY
+#endif
+
 template <bool KAHAN, bool CAPTURE>
-__global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs a) {
+__global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArgs a) {
   const GridDev& g = a.grid;
   const TreeDev& tr = a.tree;
   const RateTables& rt = a.rates;
@@ -288,7 +320,7 @@ __global__ __launch_bounds__(64) void fused_step_kernel(FusedArgs a) {
         double res[6];
         if (fast_rate_ok(rt, rr)) {
           const float* sp = tr.spec + (int64_t)u * rt.ngrid;
-          rate_loop_f32(rt, a.dsig, rr, tr.spec_scale[u], [&](int k) { return sp[k]; }, res);
+          rate_loop_f32(rt, rr, ldc(tr.spec_scale + u), [&](int k) { return ldc(sp + k); }, res);
         } else {
           const double* sp = tr.spec64 + (int64_t)u * rt.ngrid;
           rate_loop_f64(rt, rr, [&](int k) { return sp[k]; }, res);
@@ -375,12 +407,11 @@ hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, i
   return hipGetLastError();
 }
 
-hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const float* dsig,
-                            const void* mesh, const void* rays, int64_t nray, double* per_ray,
+hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* mesh, const void* rays, int64_t nray, double* per_ray,
                             int64_t* target, hipStream_t s) {
   if (nray <= 0) return hipSuccess;
   ray_rates_kernel<<<nblocks(nray, 128), 128, 0, s>>>(
-      g, rt, dsig, static_cast<const unsigned char*>(mesh),
+      g, rt, static_cast<const unsigned char*>(mesh),
       static_cast<const unsigned char*>(rays), nray, per_ray, target);
   return hipGetLastError();
 }

@@ -75,7 +75,7 @@ struct Ctx {
   // frequency tables
   DevBuf<float> nu, cs, sig, hsig, dsig;
   double cs_max[3] = {0, 0, 0}, cs_last[3] = {0, 0, 0};
-  int ngrid = 0;
+  int ngrid = 0, k_he1 = 0, k_he2 = 0;
   // tree
   DevBuf<NodeLink> link;
   DevBuf<float> cm, spec, src_pos;
@@ -129,6 +129,9 @@ RateTables rate_tables() {
   t.cs = g.cs.p;
   t.sig = g.sig.p;
   t.hsig = g.hsig.p;
+  t.dsig = g.dsig.p;
+  t.k_he1 = g.k_he1;
+  t.k_he2 = g.k_he2;
   for (int s = 0; s < 3; s++) {
     t.cs_max[s] = g.cs_max[s];
     t.cs_last[s] = g.cs_last[s];
@@ -159,6 +162,14 @@ void upload_tables(const run_param* r) {
     for (int k = 0; k < ng; k++)
       dsig[s * ng + k] = (float)((double)sig[s * ng + k] - g.cs_last[s]);
   }
+  // Bin ranges with absent species (sigma and heating weight both zero).
+  auto first_nonzero = [&](int s) {
+    for (int k = 0; k < ng; k++)
+      if (sig[s * ng + k] != 0.0f || hsig[s * ng + k] != 0.0f) return k;
+    return ng;
+  };
+  g.k_he2 = first_nonzero(2);
+  g.k_he1 = std::min(first_nonzero(1), g.k_he2);
   g.ngrid = ng;
   HIP_OK(g.nu.reserve(ng));
   HIP_OK(g.cs.reserve(3 * ng));
@@ -406,8 +417,7 @@ void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* 
   HIP_OK(g.target.reserve(nray));
   HIP_OK(hipMemcpyAsync(g.rays.p, ray_IO, bytes, hipMemcpyHostToDevice, g.stream));
   const RateTables rt = rate_tables();
-  HIP_OK(launch_ray_rates(gd, rt, g.dsig.p, g.mesh, g.rays.p, nray, g.per_ray.p, g.target.p,
-                          g.stream));
+  HIP_OK(launch_ray_rates(gd, rt, g.mesh, g.rays.p, nray, g.per_ray.p, g.target.p, g.stream));
   HIP_OK(launch_merge_ray_rates(g.per_ray.p, g.target.p, nray, g.mesh, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
 }
@@ -514,7 +524,6 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   a.tree.spec64 = g.spec64.p;
   a.tree.src_pos = g.src_pos.p;
   a.rates = rate_tables();
-  a.dsig = g.dsig.p;
   a.nf = g.nf.p;
   a.mesh = g.mesh;
   a.ctr = g.ctr.p;
