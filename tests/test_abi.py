@@ -1,0 +1,70 @@
+"""The C-ABI library (CPU, no GPU calls): it loads, exports every function
+include/argot_hip.h declares, and its structs have the reference's layout."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from argot_amd import abi
+from argot_amd._lib import EXPORTS, LIB_PATH, lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "argot_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[a-z][\w \*]*?\b(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "sizeof")))
+
+
+def test_library_exports_every_declared_symbol():
+    L = lib()
+    decl = declared_functions()
+    assert len(decl) >= 20
+    for name in decl + EXPORTS:
+        assert hasattr(L, name), name
+
+
+def test_struct_layout_matches_library():
+    n = 32
+    out = (C.c_int64 * n)()
+    m = lib().argot_hip_abi_layout(out, n)
+    assert list(out[:m]) == abi.layout_expected()
+
+
+def test_reference_layout_sizes():
+    """Sizes and offsets of the reference structs (gcc x86-64, __HELIUM__,
+    NGRID_NU = 256), as the reference's own code sees them."""
+    assert C.sizeof(abi.FluidMesh) == 156
+    assert abi.FluidMesh.prev_chem.offset == 96
+    assert C.sizeof(abi.RunParam) == 5480
+    assert abi.RunParam.delta_x.offset == 5456
+    assert C.sizeof(abi.RaySegment) == 56
+    assert C.sizeof(abi.LightRay) == 2384
+    assert C.sizeof(abi.LightRayIO) == 2096
+
+
+def test_gpu_entry_points_fail_cleanly_without_init():
+    """Entry points report ARGOT_ERR_STATE instead of aborting when the
+    device context does not exist (no HIP call is made)."""
+    L = lib()
+    run = abi.RunParam()
+    cm = abi.CudaMemSpace()
+    cp = abi.CudaParam()
+    L.zero_out_photoion_rate_cuda(C.byref(cm), C.byref(cp), C.byref(run))
+    assert L.argot_hip_last_error() == 3
+    assert L.argot_hip_error_string(3).decode().startswith("init_gpu")
+
+
+def test_host_tree_handles_tiny_inputs():
+    L = lib()
+    box = np.array([0, 0, 0, 1, 1, 1], np.float32)
+    pos = np.array([0.5, 0.5, 0.5], np.float32)
+    spec = np.ones(256)
+    h = L.argot_hip_tree_build(pos.ctypes.data, spec.ctypes.data, 1, 256, box.ctypes.data, 0.7)
+    assert h
+    t = np.array([0.1, 0.2, 0.3], np.float32)
+    assert L.argot_hip_tree_rays_to(h, t.ctypes.data, 0, None, None, None) == 1
+    L.argot_hip_tree_free(h)
