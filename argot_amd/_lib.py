@@ -65,6 +65,8 @@ def lib():
     L.argot_hip_set_theta.argtypes = [C.c_float]
     L.argot_hip_set_blocks_per_cu.argtypes = [C.c_int]
     L.argot_hip_set_capture.argtypes = [C.c_void_p, C.c_int64]
+    L.argot_hip_copy_probe.argtypes = [C.c_int64, C.c_int]
+    L.argot_hip_copy_probe.restype = C.c_double
     L.argot_hip_record_size.restype = C.c_int64
     L.argot_hip_abi_layout.argtypes = [P(C.c_int64), C.c_int]
     L.argot_hip_abi_layout.restype = C.c_int

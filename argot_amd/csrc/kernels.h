@@ -28,6 +28,8 @@ hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* 
 hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
                                   void* mesh, hipStream_t s);
 hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s);
+hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblock,
+                             hipStream_t s);
 hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int nblock,
                              hipStream_t s);
 

@@ -365,6 +365,21 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   }
 }
 
+// Plain device-to-device copy (16 B per thread per iteration, grid-stride):
+// the bandwidth probe whose byte count calibrates the HBM counters.
+__global__ void copy_probe_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                  int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblock,
+                             hipStream_t s) {
+  copy_probe_kernel<<<nblock, 256, 0, s>>>(src, dst, n);
+  return hipGetLastError();
+}
+
 __global__ void reset_counters_kernel(StepCounters* ctr) {
   const int i = threadIdx.x;
   if (i < kNumQueues) ctr->queue[i] = 0;

@@ -555,6 +555,49 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   this_run->nseg = hc.nray;
 }
 
+// Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
+// streaming kernel; returns the best GB/s (read + write bytes / time).
+double argot_hip_copy_probe(int64_t nbytes, int reps) {
+  t_err = ARGOT_OK;
+  if (nbytes < 16 || reps < 1) {
+    set_err(ARGOT_ERR_ARG);
+    return 0.0;
+  }
+  const int64_t n = nbytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 0.0;
+  int ncu = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) ncu = prop.multiProcessorCount;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess ||
+      hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    set_err(ARGOT_ERR_HIP);
+  } else if (hipMemsetAsync(a, 0, n * 16, s) == hipSuccess) {
+    for (int r = 0; r < reps && !t_err; r++) {
+      if (hipEventRecord(e0, s) != hipSuccess ||
+          launch_copy_probe(a, b, n, ncu * 8, s) != hipSuccess ||
+          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        set_err(ARGOT_ERR_HIP);
+        break;
+      }
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms > 0.0f)
+        best = std::max(best, 2.0 * (double)(n * 16) / (ms * 1e-3) / 1e9);
+    }
+  } else {
+    set_err(ARGOT_ERR_HIP);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return best;
+}
+
 // ------------------------------------------------------------------------
 // Host-only helpers for tests (no GPU needed).
 

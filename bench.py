@@ -108,6 +108,20 @@ def main():
     kernel_ms = float(np.mean(ker))
     bytes_launch = algorithmic_bytes(nray, ncross, ncell)
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
+    # memory traffic of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/traffic.json)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and not args.kahan:
+        with open(tpath) as f:
+            rec = json.load(f).get("fused_step_kernel", {})
+        if rec.get("workload") == args.workload:
+            traffic = rec.get("traffic_bytes")
+    from argot_amd._lib import check, lib
+    copy_gbs = lib().argot_hip_copy_probe(1 << 30, 10)
+    check("argot_hip_copy_probe")
+    print(f"# copy probe: {copy_gbs:.0f} GB/s (1 GiB copy, read + write)", file=sys.stderr,
+          flush=True)
     out = {
         "metric": "ray segments/s (point-source step, exact rates)",
         "value": nray / (ms_step * 1e-3),
@@ -126,9 +140,10 @@ def main():
                    "segments_per_step": int(nray), "crossings_per_step": int(ncross),
                    "tau_sum": "kahan" if args.kahan else "plain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "fused_step_kernel", "kernel_ms": kernel_ms,
-                     "bytes_per_launch": int(bytes_launch)},
+                     "bytes_per_launch": int(bytes_launch),
+                     "measured_copy_gbs": copy_gbs},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)

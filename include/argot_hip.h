@@ -290,6 +290,16 @@ struct argot_step_stats {
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
 
+/* Opening angle for step_radiation_tree (default 0.7 = ARGOT_THETA_CRIT). */
+void argot_hip_set_theta(float theta);
+
+/* Persistent grid of the fused kernel, in wavefronts per compute unit. */
+void argot_hip_set_blocks_per_cu(int n);
+
+/* Measured device copy bandwidth: best of reps copies of nbytes with a
+ * plain streaming kernel, in GB/s of bytes read plus written. */
+double argot_hip_copy_probe(int64_t nbytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif
