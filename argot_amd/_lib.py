@@ -9,7 +9,9 @@ import os
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libargot_hip.so")
+# ARGOT_HIP_LIB selects another build of the same library (e.g. a tuning
+# variant); default: the in-tree build.
+LIB_PATH = os.environ.get("ARGOT_HIP_LIB", os.path.join(HERE, "libargot_hip.so"))
 
 # Entry points that include/argot_hip.h declares (checked by tests).
 EXPORTS = [
