@@ -23,6 +23,11 @@ EXPORTS = [
     "merge_photoion_data", "step_radiation_tree",
     "argot_hip_last_error", "argot_hip_error_string", "argot_hip_set_helium_fact",
     "argot_hip_set_tau_mode", "argot_hip_set_rate_mode", "argot_hip_step_stats",
+    "argot_hip_set_theta", "argot_hip_set_blocks_per_cu", "argot_hip_copy_probe",
+    "argot_hip_spectral_grid", "argot_hip_surrogate_train", "argot_hip_surrogate_samples",
+    "argot_hip_surrogate_get_params", "argot_hip_surrogate_set_params",
+    "argot_hip_surrogate_eval", "argot_hip_surrogate_gradient",
+    "argot_hip_surrogate_trust_error",
 ]
 
 _lib = None
@@ -82,6 +87,17 @@ def lib():
     L.argot_hip_tree_rays_to.restype = C.c_uint64
     L.argot_hip_tree_spec.argtypes = [C.c_void_p, C.c_int]
     L.argot_hip_tree_spec.restype = P(C.c_double)
+    L.argot_hip_spectral_grid.argtypes = [C.c_int, C.c_double, C.c_double, C.c_float,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]
+    L.argot_hip_surrogate_train.argtypes = [P(abi.SurrogateConfig), P(abi.SurrogateReport)]
+    L.argot_hip_surrogate_samples.argtypes = [P(abi.SurrogateConfig), C.c_int64, C.c_void_p,
+                                              C.c_void_p]
+    L.argot_hip_surrogate_get_params.argtypes = [C.c_void_p, C.c_void_p]
+    L.argot_hip_surrogate_set_params.argtypes = [C.c_void_p, C.c_void_p]
+    L.argot_hip_surrogate_eval.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.argot_hip_surrogate_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                               C.c_void_p]
+    L.argot_hip_surrogate_trust_error.argtypes = [C.c_void_p, C.c_int64]
     _lib = L
     return L
 

@@ -119,14 +119,32 @@ class CudaMemSpace(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [("ntarget", C.c_uint64), ("nray", C.c_uint64), ("ncross", C.c_uint64),
                 ("nfallback", C.c_uint64), ("ms_build", C.c_double), ("ms_kernel", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("nsurrogate", C.c_uint64)]
+
+
+class SurrogateConfig(C.Structure):
+    _fields_ = [("ngrid", C.c_int), ("nu", C.POINTER(C.c_double)), ("cs", C.POINTER(C.c_float)),
+                ("spec", C.POINTER(C.c_double)), ("nsample", C.c_int64), ("batch", C.c_int),
+                ("nstep", C.c_int), ("lr_max", C.c_float), ("lr_min", C.c_float),
+                ("seed", C.c_uint64), ("log_tau_lo", C.c_float), ("log_tau_hi", C.c_float),
+                ("trust_n", C.c_int), ("trust_tol", C.c_float), ("dtau_max", C.c_float)]
+
+
+class SurrogateReport(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double),
+                ("ms_samples", C.c_double), ("ms_train", C.c_double), ("ms_validate", C.c_double),
+                ("err_max", C.c_double), ("err_p99", C.c_double), ("err_median", C.c_double),
+                ("trusted_frac", C.c_double)]
+
+
+SURROGATE_NPARAM = 4806
 
 
 # One captured ray (test mode of step_radiation_tree), 120 bytes.
 RAY_RECORD = np.dtype([
     ("target", np.int64), ("seq", np.int32), ("kind", np.int32), ("id", np.int32),
     ("ncross", np.int32), ("start", np.float32, 3), ("src", np.float32, 3),
-    ("tau", np.float32, 3), ("pad", np.float32), ("hash", np.uint64), ("rate", np.float64, 6)])
+    ("tau", np.float32, 3), ("surrogate", np.float32), ("hash", np.uint64), ("rate", np.float64, 6)])
 
 
 def layout_expected():

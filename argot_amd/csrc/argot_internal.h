@@ -77,7 +77,7 @@ struct RayRecord {
   float start[3];       // segment start (global coordinates)
   float src[3];         // ray source position
   float tau[3];         // FP32 optical depths (code units)
-  float pad;
+  float surrogate;      // 1: rates from the network, 0: exact integral
   uint64_t hash;        // FNV-1a of the visited cell sequence
   double rate[6];       // this ray's rates (heats times hplanck*nuL)
 };
@@ -87,6 +87,7 @@ struct StepCounters {
   unsigned long long ncross;
   unsigned long long nfallback;
   unsigned long long nrecord;
+  unsigned long long nsurrogate;
   int queue[kNumQueues];
 };
 

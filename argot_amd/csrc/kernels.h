@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "argot_internal.h"
+#include "surrogate.h"
 
 namespace argot {
 
@@ -15,6 +16,7 @@ struct FusedArgs {
   StepCounters* ctr;
   RayRecord* rec;              // capture mode only
   int64_t rec_cap;
+  SurrDev surr;                // rate surrogate (surrogate launches only)
 };
 
 hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fact, float4* nf,
@@ -30,7 +32,7 @@ hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, 
 hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s);
 hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblock,
                              hipStream_t s);
-hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int nblock,
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,
                              hipStream_t s);
 
 }  // namespace argot

@@ -254,7 +254,65 @@ __device__ __forceinline__ int64_t next_tile(StepCounters* ctr, int64_t ntile, i
 #define ARGOT_OCCUPANCY
 #endif
 
-template <bool KAHAN, bool CAPTURE>
+// 2-point Gauss-Legendre nodes on [0, 1].
+constexpr double kGl1 = 0.21132486540518711775, kGl2 = 0.78867513459481288225;
+
+// Can the network serve this ray?  Its node's spectrum must have the
+// trained shape (q > 0), the in-cell optical depth must be small enough
+// for the 2-point rule, and both nodes of the in-cell path (threshold
+// optical depths (t_s + x d_s) sig_th_s) must lie below the hull's upper
+// edge and in trusted cells.  Below the lower edge the kernels are flat
+// (relative change < 1e-5), so inputs are clamped there.
+__device__ inline bool surr_inputs(const SurrDev& S, const RateTables& rt, const RayRate& rr,
+                                   double q, float xa[3], float xb[3]) {
+  if (!(q > 0.0)) return false;
+  bool ok = true;
+  double dmax = 0.0;
+  int ca[3], cb[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const double t = rr.t[s], d = rr.d[s];
+    ok = ok && t >= 0.0 && d >= 0.0;
+    dmax += d * rt.cs_max[s];
+    const double ta = (t + kGl1 * d) * (double)S.sig_th[s];
+    const double tb = (t + kGl2 * d) * (double)S.sig_th[s];
+    ok = ok && tb <= S.tau_hi;
+    xa[s] = __log10f((float)fmax(ta, S.tau_lo));
+    xb[s] = __log10f((float)fmax(tb, S.tau_lo));
+    ca[s] = min(S.trust_n - 1, max(0, (int)((xa[s] - S.lo) * S.inv_cell)));
+    cb[s] = min(S.trust_n - 1, max(0, (int)((xb[s] - S.lo) * S.inv_cell)));
+  }
+  ok = ok && dmax <= (double)S.dtau_max;
+  if (!ok) return false;
+  const int n = S.trust_n;
+  return S.trust[(ca[0] * n + ca[1]) * n + ca[2]] && S.trust[(cb[0] * n + cb[1]) * n + cb[2]];
+}
+
+// 10^y in FP64 from an FP32 exponent (kernels reach 1e-90, below FP32).
+__device__ __forceinline__ double exp10_wide(float y) {
+  const double t = (double)y * 3.3219280948873623479;   // log2(10)
+  const double n = floor(t);
+  return ldexp((double)exp2f((float)(t - n)), (int)n);
+}
+
+// Rates of a ray from the network: Gamma_s = r_s d_s q <F_s>, Heat_s =
+// r_s d_s q <G_s> hplanck nu_L, with <.> the 2-point mean over the in-cell
+// path (the photon-conserving cell average of exp(-tau)).
+__device__ inline void surr_rates(const RayRate& rr, double q, const float ya[kNout],
+                                  const float yb[kNout], double res[6]) {
+  double m[kNout];
+#pragma unroll
+  for (int o = 0; o < kNout; o++) m[o] = 0.5 * (exp10_wide(ya[o]) + exp10_wide(yb[o]));
+  const double p0 = rr.r[0] * rr.d[0] * q, p1 = rr.r[1] * rr.d[1] * q, p2 = rr.r[2] * rr.d[2] * q;
+  res[0] = p0 * m[0];
+  res[1] = p0 * m[3] * kHnuL;
+  res[2] = p1 * m[1];
+  res[3] = p2 * m[2];
+  res[4] = p1 * m[4] * kHnuL;
+  res[5] = p2 * m[5] * kHnuL;
+}
+
+template <bool KAHAN, bool CAPTURE, bool SURR>
 __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArgs a) {
   const GridDev& g = a.grid;
   const TreeDev& tr = a.tree;
@@ -264,7 +322,12 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
             ntz = (g.nz + kTile - 1) / kTile;
   const int64_t ntile = (int64_t)ntx * nty * ntz;
   int q = blockIdx.x % kNumQueues, tried = 0;
-  unsigned long long my_ray = 0, my_cross = 0, my_fb = 0;
+  unsigned long long my_ray = 0, my_cross = 0, my_fb = 0, my_surr = 0;
+  float* lds = nullptr;
+  if constexpr (SURR) {
+    __shared__ float surr_lds[kMlpLds];
+    lds = surr_lds;
+  }
 
   for (;;) {
     const int64_t t = next_tile(a.ctr, ntile, &q, &tried);
@@ -303,21 +366,40 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
       const bool leaf = ln.n <= 1;
       const bool emit = here && (accept || ln.n == 1);
       if (here) p = (accept || leaf || ln.first < 0) ? ln.skip : ln.first;
+      Vec3f src{0.0f, 0.0f, 0.0f}, st{0.0f, 0.0f, 0.0f};
+      WalkOut w{};
+      RayRate rr{};
+      double res[6] = {0, 0, 0, 0, 0, 0};
       if (emit) {
-        Vec3f src;
         if (accept) {
           src = Vec3f{cx, cy, cz};
         } else {
           src = Vec3f{tr.src_pos[4 * ln.src + 0], tr.src_pos[4 * ln.src + 1],
                       tr.src_pos[4 * ln.src + 2]};
         }
-        Vec3f st;
         segment_start(g, src, tgt, &st);
-        WalkOut w;
         walk_segment<KAHAN, CAPTURE>(g, a.nf, st, tgt, &w);
-        RayRate rr;
         ray_rate_setup(g, rt, w.tau, src, tgt, dens, fHI, fHeI, fHeII, &rr);
-        double res[6];
+      }
+      bool exact = emit;
+      if constexpr (SURR) {
+        // The network runs on the whole wavefront (MFMA) when any lane
+        // can use it; the other lanes take the exact integral below.
+        const double qn = a.surr.node_q[u];
+        float xa[3] = {0.0f, 0.0f, 0.0f}, xb[3] = {0.0f, 0.0f, 0.0f};
+        const bool use = emit && surr_inputs(a.surr, rt, rr, qn, xa, xb);
+        if (__ballot(use) != 0) {
+          float ya[kNout], yb[kNout];
+          mlp_forward(a.surr.net, xa[0], xa[1], xa[2], ya, lds);
+          mlp_forward(a.surr.net, xb[0], xb[1], xb[2], yb, lds);
+          if (use) {
+            surr_rates(rr, qn, ya, yb, res);
+            exact = false;
+            my_surr++;
+          }
+        }
+      }
+      if (exact) {
         if (fast_rate_ok(rt, rr)) {
           const float* sp = tr.spec + (int64_t)u * rt.ngrid;
           rate_loop_f32(rt, rr, ldc(tr.spec_scale + u), [&](int k) { return ldc(sp + k); }, res);
@@ -326,6 +408,8 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
           rate_loop_f64(rt, rr, [&](int k) { return sp[k]; }, res);
           my_fb++;
         }
+      }
+      if (emit) {
 #pragma unroll
         for (int k = 0; k < 6; k++) acc[k] += res[k];
         my_ray++;
@@ -342,7 +426,7 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
             R.start[0] = st.x; R.start[1] = st.y; R.start[2] = st.z;
             R.src[0] = src.x; R.src[1] = src.y; R.src[2] = src.z;
             R.tau[0] = w.tau[0]; R.tau[1] = w.tau[1]; R.tau[2] = w.tau[2];
-            R.pad = 0.0f;
+            R.surrogate = exact ? 0.0f : 1.0f;
             R.hash = w.hash;
             for (int k = 0; k < 6; k++) R.rate[k] = res[k];
           }
@@ -357,11 +441,12 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     }
   }
   const unsigned long long sr = wave_sum(my_ray), sc = wave_sum(my_cross),
-                           sf = wave_sum(my_fb);
+                           sf = wave_sum(my_fb), ss = wave_sum(my_surr);
   if (lane == 0) {
     atomicAdd(&a.ctr->nray, sr);
     atomicAdd(&a.ctr->ncross, sc);
     atomicAdd(&a.ctr->nfallback, sf);
+    if (SURR) atomicAdd(&a.ctr->nsurrogate, ss);
   }
 }
 
@@ -388,6 +473,7 @@ __global__ void reset_counters_kernel(StepCounters* ctr) {
     ctr->ncross = 0;
     ctr->nfallback = 0;
     ctr->nrecord = 0;
+    ctr->nsurrogate = 0;
   }
 }
 
@@ -444,19 +530,27 @@ hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int nblock,
-                             hipStream_t s) {
+template <bool SURR>
+static void launch_fused(const FusedArgs& a, int kahan, int capture, int nblock, hipStream_t s) {
   if (capture) {
     if (kahan)
-      fused_step_kernel<true, true><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, true, SURR><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, true><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, true, SURR><<<nblock, 64, 0, s>>>(a);
   } else {
     if (kahan)
-      fused_step_kernel<true, false><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, false, SURR><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, false><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, false, SURR><<<nblock, 64, 0, s>>>(a);
   }
+}
+
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,
+                             hipStream_t s) {
+  if (surr)
+    launch_fused<true>(a, kahan, capture, nblock, s);
+  else
+    launch_fused<false>(a, kahan, capture, nblock, s);
   return hipGetLastError();
 }
 

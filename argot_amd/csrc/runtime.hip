@@ -13,6 +13,7 @@
 #include "argot_hip.h"
 #include "argot_internal.h"
 #include "kernels.h"
+#include "surrogate.h"
 #include "tree.h"
 
 using namespace argot;
@@ -81,6 +82,7 @@ struct Ctx {
   DevBuf<float> cm, spec, src_pos;
   DevBuf<double> spec_scale, spec64;
   DevBuf<StepCounters> ctr;
+  DevBuf<double> node_q;          // surrogate: photon rate of conforming nodes
   DevBuf<RayRecord> rec;
   // segment / ray staging for the per-call entry points
   DevBuf<unsigned char> seg, rays;
@@ -209,6 +211,11 @@ double now_ms() {
 
 }  // namespace
 
+namespace argot {
+void argot_hip_internal_set_error(int code) { set_err(code); }
+void argot_hip_internal_clear_error() { t_err = ARGOT_OK; }
+}  // namespace argot
+
 extern "C" {
 
 int argot_hip_last_error(void) { return t_err; }
@@ -218,7 +225,7 @@ const char* argot_hip_error_string(int code) {
     case ARGOT_OK: return "ok";
     case ARGOT_ERR_HIP: return "HIP runtime call failed";
     case ARGOT_ERR_ARG: return "bad argument or size";
-    case ARGOT_ERR_STATE: return "init_gpu / send_mesh_data not called";
+    case ARGOT_ERR_STATE: return "init_gpu / send_mesh_data not called, or no trained surrogate";
     case ARGOT_ERR_NOMEM: return "device allocation failed";
     case ARGOT_ERR_TOO_MANY_SEGMENTS: return "ray crosses too many domains";
     default: return "unknown";
@@ -524,6 +531,21 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   a.tree.spec64 = g.spec64.p;
   a.tree.src_pos = g.src_pos.p;
   a.rates = rate_tables();
+  // Rate mode 1 needs a trained network; with no trusted cell every ray
+  // would take the exact integral, so the plain kernel runs.
+  if (g_rate_mode == 1 && !surrogate_ready()) return set_err(ARGOT_ERR_STATE);
+  const int surr = g_rate_mode == 1 && surrogate_any_trusted();
+  if (surr) {
+    std::vector<double> q;
+    surrogate_node_q(T.spec64.data(), nn, ARGOT_NGRID_NU, &q);
+    HIP_OK(g.node_q.reserve(q.size()));
+    HIP_OK(hipMemcpyAsync(g.node_q.p, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice,
+                          g.stream));
+    a.surr = surrogate_dev();
+    a.surr.node_q = g.node_q.p;
+  } else {
+    a.surr = SurrDev{};
+  }
   a.nf = g.nf.p;
   a.mesh = g.mesh;
   a.ctr = g.ctr.p;
@@ -533,7 +555,7 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   a.rec_cap = capture ? g_capture_cap : 0;
   const int nblock = std::max(1, g.ncu * g_blocks_per_cu);
   HIP_OK(hipEventRecord(g.ev0, g.stream));
-  HIP_OK(launch_fused_step(a, g_tau_mode, capture, nblock, g.stream));
+  HIP_OK(launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream));
   HIP_OK(hipEventRecord(g.ev1, g.stream));
   StepCounters hc;
   HIP_OK(hipMemcpyAsync(&hc, g.ctr.p, sizeof(hc), hipMemcpyDeviceToHost, g.stream));
@@ -548,6 +570,7 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   g.stats.nray = hc.nray;
   g.stats.ncross = hc.ncross;
   g.stats.nfallback = hc.nfallback;
+  g.stats.nsurrogate = hc.nsurrogate;
   g.stats.ms_build = t_built - t_start;
   g.stats.ms_kernel = ms;
   g.stats.ms_total = now_ms() - t_start;

@@ -112,4 +112,21 @@ void setup_cross_section(struct cross_section* csect, struct freq_param* freq) {
   }
 }
 
+void argot_hip_spectral_grid(int ngrid, double log_nu_min, double log_nu_max, float temperature,
+                             double* nu, float* cs, double* spec) {
+  if (ngrid <= 0 || !(log_nu_max > log_nu_min) || !nu || !cs || !spec) return;
+  const double dl = (log_nu_max - log_nu_min) / ngrid;
+  double tot = 0.0;
+  for (int i = 0; i < ngrid; i++) {
+    nu[i] = pow(10.0, log_nu_min + (i + 0.5) * dl);
+    cs[3 * i + 0] = (float)sigma_HI(nu[i]);
+    cs[3 * i + 1] = (float)sigma_HeI(nu[i]);
+    cs[3 * i + 2] = (float)sigma_HeII(nu[i]);
+    const double lo = log_nu_min + dl * i;
+    spec[i] = integrate(planck_photons, (double)temperature, pow(10.0, lo), pow(10.0, lo + dl));
+    tot += spec[i];
+  }
+  for (int i = 0; i < ngrid; i++) spec[i] /= tot;
+}
+
 }  // extern "C"

@@ -108,9 +108,8 @@ class PointSourceStep:
         self.mesh = fill_mesh(w)
         self.cmem = abi.CudaMemSpace()
         self.cpar = abi.CudaParam()
+        self.tau_mode, self.rate_mode = tau_mode, rate_mode
         L.argot_hip_set_helium_fact(w.helium_fact)
-        L.argot_hip_set_tau_mode(tau_mode)
-        L.argot_hip_set_rate_mode(rate_mode)
         L.init_gpu(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
         check("init_gpu")
         L.send_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
@@ -118,6 +117,9 @@ class PointSourceStep:
 
     def step(self):
         L = lib()
+        L.argot_hip_set_helium_fact(self.w.helium_fact)
+        L.argot_hip_set_tau_mode(self.tau_mode)
+        L.argot_hip_set_rate_mode(self.rate_mode)
         L.step_radiation_tree(self.mesh, self.src, C.byref(self.run), None,
                               C.byref(self.cmem), C.byref(self.cpar), 0.0)
         check("step_radiation_tree")

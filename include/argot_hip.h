@@ -257,7 +257,8 @@ enum argot_hip_status {
     ARGOT_OK = 0,
     ARGOT_ERR_HIP = 1,          /* a HIP runtime call failed */
     ARGOT_ERR_ARG = 2,          /* bad argument or size */
-    ARGOT_ERR_STATE = 3,        /* init_gpu not called / mesh not sent */
+    ARGOT_ERR_STATE = 3,        /* init_gpu not called / mesh not sent /
+                                   rate mode 1 without a trained surrogate */
     ARGOT_ERR_NOMEM = 4,        /* device allocation failed */
     ARGOT_ERR_TOO_MANY_SEGMENTS = 5  /* ray crosses too many domains */
 };
@@ -274,8 +275,8 @@ void argot_hip_set_helium_fact(double helium_fact);
  * 1 = Kahan-compensated FP32 sums. */
 void argot_hip_set_tau_mode(int mode);
 
-/* Rate integral: 0 = exact frequency integral (default), 1 = MLP
- * surrogate with exact fallback outside its training range. */
+/* Rate integral of step_radiation_tree: 0 = exact frequency integral
+ * (default), 1 = MLP surrogate with exact fallback (see below). */
 void argot_hip_set_rate_mode(int mode);
 
 /* Counters of the last step_radiation_tree call. */
@@ -283,12 +284,84 @@ struct argot_step_stats {
     uint64_t ntarget;        /* target cells */
     uint64_t nray;           /* rays (== segments, one rank) */
     uint64_t ncross;         /* cell crossings incl. the end-cell term */
-    uint64_t nfallback;      /* surrogate rays that used the exact integral */
+    uint64_t nfallback;      /* rays whose rates took the FP64 reference loop */
     double ms_build;         /* host tree build + upload */
     double ms_kernel;        /* fused kernel(s), HIP events */
     double ms_total;         /* whole call, wall clock */
+    uint64_t nsurrogate;     /* rays whose rates came from the MLP surrogate */
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
+
+/* ------------------------------------------------------------------ */
+/* Rate surrogate (no reference counterpart)                            */
+/* ------------------------------------------------------------------ */
+/* A 3 -> 64 -> 64 -> 6 SiLU network with bf16 weights maps the log10
+ * threshold optical depths (HI, HeI, HeII) of a ray to log10 of the
+ * unit-spectrum kernels F_s = sum_k S_k sigma_s(k) exp(-tau_k) and
+ * G_s = sum_k S_k sigma_s(k) (nu_k - nu_s) exp(-tau_k), s = HI, HeI, HeII.
+ * The photon-conserving cell average of a ray is the 2-point
+ * Gauss-Legendre mean of the network along the in-cell optical-depth path.
+ * It is trained on the GPU (bf16 MFMA forward and backward, FP32
+ * accumulation, Adam) on samples of the exact integral over the given
+ * frequency grid and spectrum shape, then validated against that integral
+ * on a trust grid over its input hull: a ray is served by the network only
+ * if its spectrum has the trained shape, its path lies in the hull and in
+ * trusted cells, and its in-cell optical depth is within dtau_max;
+ * otherwise the exact integral runs. */
+struct argot_surrogate_config {
+    int ngrid;                 /* frequency bins of the integral */
+    const double *nu;          /* [ngrid] nu / nu_L */
+    const float *cs;           /* [3*ngrid] sigma HI, HeI, HeII per bin (cm^2) */
+    const double *spec;        /* [ngrid] photon rate per bin (shape; any scale) */
+    int64_t nsample;           /* training samples, multiple of batch */
+    int batch;                 /* samples per optimizer step, multiple of 64 */
+    int nstep;                 /* optimizer steps */
+    float lr_max, lr_min;      /* Adam step size: 2% linear warm-up, cosine decay */
+    uint64_t seed;             /* samples, initial weights and validation points */
+    float log_tau_lo, log_tau_hi;  /* input hull per axis (log10 tau) */
+    int trust_n;               /* trust cells per axis */
+    float trust_tol;           /* max relative kernel error of a trusted cell */
+    float dtau_max;            /* max in-cell optical depth for the 2-point rule */
+};
+struct argot_surrogate_report {
+    double loss_first;         /* normalised MSE of the first step's batch */
+    double loss_last;          /* mean over the last 1% of steps */
+    double ms_samples, ms_train, ms_validate;
+    double err_max, err_p99, err_median;   /* relative kernel error, 64 points per trust cell */
+    double trusted_frac;       /* fraction of trust cells within trust_tol */
+};
+/* A log frequency grid for the surrogate: ngrid bins over
+ * [10^log_nu_min, 10^log_nu_max] nu_L with centres nu[i] (the convention
+ * of setup_freq_param), cross sections cs[3*i + s] (setup_cross_section's
+ * fits) and a blackbody photon spectrum per bin (setup_photon_rate's
+ * quadrature) normalised to a unit total. */
+void argot_hip_spectral_grid(int ngrid, double log_nu_min, double log_nu_max, float temperature,
+                             double *nu, float *cs, double *spec);
+/* Generate samples, train, validate; the network then serves
+ * step_radiation_tree when argot_hip_set_rate_mode(1) is set and the
+ * step's frequency grid has ngrid bins. */
+void argot_hip_surrogate_train(const struct argot_surrogate_config *cfg,
+                               struct argot_surrogate_report *rep);
+/* The first n training samples of cfg's generator (log10 tau [n*3],
+ * log10 kernels [n*6]), without training. */
+void argot_hip_surrogate_samples(const struct argot_surrogate_config *cfg, int64_t n,
+                                 float *log_tau, float *log_kernel);
+/* FP32 master parameters (ARGOT_SURROGATE_NPARAM floats: W1[3][64],
+ * b1[64], W2[64][64], b2[64], W3[64][6], b3[6]) and normalisation
+ * (xc, xs, ymean[6], ystd[6]: input (log10 tau - xc) * xs, output
+ * log10 F = ymean + ystd * y). */
+#define ARGOT_SURROGATE_NPARAM 4806
+void argot_hip_surrogate_get_params(float *params, float *norm);
+void argot_hip_surrogate_set_params(const float *params, const float *norm);
+/* Network outputs (log10 kernels [n*6]) for log10 tau [n*3]. */
+void argot_hip_surrogate_eval(const float *log_tau, float *log_kernel, int64_t n);
+/* Gradient of the normalised mean squared error over n samples (n a
+ * multiple of 64) with respect to the parameters, as the training kernel
+ * computes it (straight through the bf16 rounding of the weights). */
+void argot_hip_surrogate_gradient(const float *log_tau, const float *log_kernel, int64_t n,
+                                  float *grad, double *loss);
+/* Validated relative error per trust cell (trust_n^3 floats, x slowest). */
+void argot_hip_surrogate_trust_error(float *err, int64_t n);
 
 /* Opening angle for step_radiation_tree (default 0.7 = ARGOT_THETA_CRIT). */
 void argot_hip_set_theta(float theta);

@@ -1,0 +1,153 @@
+"""Rate surrogate on the GPU against its FP64 oracle: the sample generator,
+the bf16-MFMA forward pass, the training kernels' gradient, a short training
+run with its validation report, and the step with rate_mode 1 against the
+FP64 rates."""
+import numpy as np
+import pytest
+
+import mlp_ref as M
+from argot_amd import radiation as R
+from argot_amd import surrogate as S
+from helpers import oracle_problem, small_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def thin_workload():
+    """16^3 lognormal mesh, mostly ionized (in-cell optical depths well
+    below one), so that many rays fall inside the network's hull."""
+    w = small_workload(n=16, nsrc=5, seed=21)
+    w.fHI *= np.float32(1e-2)
+    w.fHeI *= np.float32(1e-2)
+    return w
+
+
+def step_config(st, **kw):
+    nu, cs, spec = S.step_grid(st.freq, st.cs, np.ctypeslib.as_array(st.src[0].photon_rate))
+    return S.Config(nu, cs, spec, **kw)
+
+
+def c4_config(**kw):
+    nu, cs, spec = S.spectral_grid(128, S.C4_LOG_NU_MAX)
+    return S.Config(nu, cs, spec, **kw)
+
+
+def random_params(seed):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(scale=0.35, size=M.NPARAM).astype(np.float32)
+    norm = np.concatenate([[-1.0, 0.2], -20.0 + rng.normal(size=6), rng.uniform(1.0, 5.0, 6)])
+    return P, norm.astype(np.float32)
+
+
+def test_samples_match_oracle(require_gpu):
+    cfg = c4_config(seed=99)
+    x, y = S.samples(cfg, 4096)
+    np.testing.assert_array_equal(x, M.sample_log_tau(99, 4096, -6.0, 4.0))
+    yo = M.kernels(cfg.nu, cfg.cs, cfg.spec, x.astype(np.float64))
+    np.testing.assert_allclose(y, yo, rtol=1e-6, atol=1e-6)
+
+
+def test_params_roundtrip(require_gpu):
+    P, norm = random_params(0)
+    S.set_params(P, norm)
+    P2, norm2 = S.get_params()
+    np.testing.assert_array_equal(P2, P)
+    np.testing.assert_array_equal(norm2, norm)
+
+
+def test_forward_matches_oracle(require_gpu):
+    P, norm = random_params(1)
+    S.set_params(P, norm)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-6, 4, (1000, 3)).astype(np.float32)    # not a multiple of 64
+    y = S.evaluate(x)
+    yo = M.forward(M.quantize(P), norm, x)
+    # bf16 weights are exact; activations keep ~16 bits (hi + lo parts)
+    err = np.abs(y - yo) / norm[8:]
+    assert err.max() < 2e-4, err.max()
+
+
+def test_gradient_matches_oracle(require_gpu):
+    P, norm = random_params(2)
+    S.set_params(P, norm)
+    rng = np.random.default_rng(2)
+    n = 4096
+    x = rng.uniform(-6, 4, (n, 3)).astype(np.float32)
+    t = (norm[2:8] + norm[8:] * rng.normal(size=(n, 6))).astype(np.float32)
+    g, loss = S.gradient(x, t)
+    lo, go = M.loss_and_grad(M.quantize(P), norm, x, t)
+    assert abs(loss / lo - 1) < 1e-4
+    blocks = {"W1": (M.OFF_W1, M.OFF_B1), "b1": (M.OFF_B1, M.OFF_W2),
+              "W2": (M.OFF_W2, M.OFF_B2), "b2": (M.OFF_B2, M.OFF_W3),
+              "W3": (M.OFF_W3, M.OFF_B3), "b3": (M.OFF_B3, M.NPARAM)}
+    for name, (a, b) in blocks.items():
+        rel = np.linalg.norm(g[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        assert rel < 2e-3, (name, rel)
+
+
+def test_training_reduces_loss_and_report_is_honest(require_gpu):
+    cfg = c4_config(nsample=1 << 18, batch=4096, nstep=3000, trust_n=8, seed=5)
+    rep = S.train(cfg)
+    assert rep["loss_last"] < 0.03 * rep["loss_first"], rep
+    err = S.trust_error(8)
+    assert np.isclose(err.max(), rep["err_max"], rtol=1e-6)
+    assert np.isclose(np.mean(err <= 1e-3), rep["trusted_frac"])
+    # independent validation points, oracle kernels
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-6, 4, (4096, 3)).astype(np.float32)
+    rel = np.abs(10.0 ** (S.evaluate(x).astype(np.float64)
+                          - M.kernels(cfg.nu, cfg.cs, cfg.spec, x.astype(np.float64))) - 1)
+    assert 0.3 < np.median(rel.max(1)) / rep["err_median"] < 3.0, (np.median(rel), rep)
+
+
+def test_step_without_trusted_cells_equals_exact(require_gpu):
+    """At trust_tol 1e-3 no cell of this network qualifies: every ray takes
+    the exact integral and the rates equal the plain path's bit for bit."""
+    w = thin_workload()
+    st = R.PointSourceStep(w, rate_mode=1)
+    rep = S.train(step_config(st, nsample=1 << 20, batch=16384, nstep=4000, trust_n=16,
+                              log_tau_lo=-3.0, log_tau_hi=2.5, trust_tol=1e-3))
+    assert rep["trusted_frac"] == 0.0, rep
+    s1 = st.step()
+    r1 = st.rates()
+    ex = R.PointSourceStep(w, rate_mode=0)
+    ex.step()
+    assert s1["nsurrogate"] == 0
+    np.testing.assert_array_equal(r1, ex.rates())
+
+
+def test_step_serves_trusted_rays_within_validated_error(require_gpu):
+    """With a loose tolerance the network serves the rays whose path lies in
+    trusted cells; those rays match the FP64 rates to the validated error,
+    and every other ray is the exact path's ray bit for bit."""
+    tol = 0.1
+    w = thin_workload()
+    st = R.PointSourceStep(w, rate_mode=1)
+    rep = S.train(step_config(st, nsample=1 << 20, batch=16384, nstep=20000, trust_n=16,
+                              log_tau_lo=-3.0, log_tau_hi=2.5, trust_tol=tol))
+    assert rep["trusted_frac"] > 0.2, rep
+    cap = 1 << 18
+    rec, stats = st.capture(cap)
+    assert stats["nray"] <= cap
+    served = rec["surrogate"] == 1.0
+    assert served.sum() == stats["nsurrogate"] > 0
+    ex = R.PointSourceStep(w, rate_mode=0)
+    rec0, _ = ex.capture(cap)
+    np.testing.assert_array_equal(rec0["target"], rec["target"])
+    np.testing.assert_array_equal(rec0["tau"], rec["tau"])
+    np.testing.assert_array_equal(rec0["rate"][~served], rec["rate"][~served])
+    P = oracle_problem(w)
+    n = w.n
+    idx = np.nonzero(served)[0]
+    rng = np.random.default_rng(0)
+    idx = rng.choice(idx, min(len(idx), 3000), replace=False)
+    ref = np.array([P.ray_rate(rec["src"][i], P.ray_spec(int(rec["kind"][i]), int(rec["id"][i])),
+                               int(rec["target"][i]) // (n * n), (int(rec["target"][i]) // n) % n,
+                               int(rec["target"][i]) % n, rec["tau"][i].astype(np.float64),
+                               mode=1) for i in idx])
+    got = rec["rate"][idx]
+    floor = 1e-6 * np.abs(ref).max(0)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), floor)
+    # validated error of the trust cells plus the 2-point rule and the
+    # interpolation between validation points
+    assert rel.max() < 3 * tol, rel.max(0)
