@@ -28,6 +28,10 @@ EXPORTS = [
     "argot_hip_surrogate_get_params", "argot_hip_surrogate_set_params",
     "argot_hip_surrogate_eval", "argot_hip_surrogate_gradient",
     "argot_hip_surrogate_trust_error",
+    "init_hd_param", "free_hd_param", "setup_diffuse_photon", "init_diffuse_gpu",
+    "send_diffuse_data", "calc_diffuse_photon_radiation", "free_diffuse_gpu",
+    "argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
+    "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
 ]
 
 _lib = None
@@ -98,6 +102,18 @@ def lib():
     L.argot_hip_surrogate_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                                C.c_void_p]
     L.argot_hip_surrogate_trust_error.argtypes = [C.c_void_p, C.c_int64]
+    hd, cd = P(abi.HostDiffuseParam), C.c_void_p
+    L.init_hd_param.argtypes = [hd]
+    L.free_hd_param.argtypes = [hd]
+    L.setup_diffuse_photon.argtypes = [hd, rp]
+    L.init_diffuse_gpu.argtypes = [cd, cp, rp]
+    L.send_diffuse_data.argtypes = [hd, cd, cp]
+    L.calc_diffuse_photon_radiation.argtypes = [C.c_void_p, rp, cm, cp, hd, cd, C.c_void_p]
+    L.free_diffuse_gpu.argtypes = [cd, cp]
+    for n in ("argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
+              "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan"):
+        getattr(L, n).argtypes = [C.c_int]
+    L.argot_hip_diffuse_stats.argtypes = [P(abi.DiffuseStats)]
     _lib = L
     return L
 

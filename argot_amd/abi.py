@@ -170,3 +170,27 @@ def mesh_array(n_cells):
 def mesh_view(arr):
     """numpy structured view (no copy) of a ctypes FluidMesh array."""
     return np.ctypeslib.as_array(arr)
+
+
+# ART diffuse photons (include/argot_hip.h; __HELIUM__ on, __HELIUM_BB__ off)
+class AngleInfo(C.Structure):
+    _fields_ = [("xovr", C.c_float), ("yovr", C.c_float), ("zovr", C.c_float),
+                ("base_id", C.c_short), ("corner_id", C.c_short)]
+
+
+class StepFuncFactor(C.Structure):
+    _fields_ = [("HI", C.c_float * 2), ("HeI", C.c_float * 2), ("HeII", C.c_float * 2)]
+
+
+class HostDiffuseParam(C.Structure):
+    _fields_ = [("corner_id_num", C.c_int * 8), ("step_fact", C.POINTER(StepFuncFactor)),
+                ("angle", C.POINTER(AngleInfo))]
+
+
+class CudaDiffuseParam(C.Structure):
+    _fields_ = [("step_fact", C.c_void_p), ("angle", C.c_void_p), ("rmesh", C.c_void_p)]
+
+
+class DiffuseStats(C.Structure):
+    _fields_ = [("ndir", C.c_uint64), ("ncross", C.c_uint64), ("ms_trace", C.c_double),
+                ("ms_gh", C.c_double), ("ms_total", C.c_double)]

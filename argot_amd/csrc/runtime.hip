@@ -214,6 +214,7 @@ double now_ms() {
 namespace argot {
 void argot_hip_internal_set_error(int code) { set_err(code); }
 void argot_hip_internal_clear_error() { t_err = ARGOT_OK; }
+double argot_hip_internal_helium_fact() { return g_helium_fact; }
 }  // namespace argot
 
 extern "C" {

@@ -29,6 +29,8 @@ bool surrogate_any_trusted();
 // Status of the calling thread's last entry point (runtime.hip).
 void argot_hip_internal_set_error(int code);
 void argot_hip_internal_clear_error();
+// n_He/n_H of argot_hip_set_helium_fact (runtime.hip).
+double argot_hip_internal_helium_fact();
 // Device view (node_q left null; the runtime fills it in).
 SurrDev surrogate_dev();
 // For every tree node (spectra spec64[nnode*ngrid]): its total photon rate

@@ -138,4 +138,30 @@ def clumped256(seed=3, n=256, nsrc=4096, nclump=8, sigma_cells=8.0):
                           "sigma_cells": sigma_cells})
 
 
-WORKLOADS = {"uniform64": uniform64, "lognormal128": lognormal128, "clumped256": clumped256}
+def ionized_fractions(rng, shape):
+    """Seeded ionization state for the diffuse transfer: x_HII ~ U[0,1];
+    helium a ~ U[0,1] ionized once, b ~ U[0,1] of that twice."""
+    x = rng.uniform(0.0, 1.0, shape)
+    a = rng.uniform(0.0, 1.0, shape)
+    b = rng.uniform(0.0, 1.0, shape)
+    fHII = x.astype(np.float32)
+    fHI = (1.0 - x).astype(np.float32)
+    fHeI = (1.0 - a).astype(np.float32)
+    fHeII = (a * (1.0 - b)).astype(np.float32)
+    fHeIII = (a * b).astype(np.float32)
+    return fHI, fHII, fHeI, fHeII, fHeIII
+
+
+def art128(seed=5, n=128):
+    """Diffuse (ART) workload: the lognormal field of lognormal128's recipe,
+    seeded ionized fractions, T = 1e4 K, no point sources."""
+    rng = np.random.default_rng(seed)
+    dens = lognormal_density(n, seed + 1000)
+    fHI, fHII, fHeI, fHeII, fHeIII = ionized_fractions(rng, dens.shape)
+    return Workload(f"art{n}", n, dens, fHI, fHII, fHeI, fHeII, fHeIII,
+                    np.zeros((0, 3), np.float32), np.zeros(0), np.zeros(0, np.float32),
+                    meta={"seed": seed})
+
+
+WORKLOADS = {"uniform64": uniform64, "lognormal128": lognormal128, "clumped256": clumped256,
+             "art128": art128}

@@ -250,6 +250,82 @@ void step_radiation_tree(struct fluid_mesh *mesh, struct radiation_src *src,
                          struct cuda_param *this_cuda, float dtime);
 
 /* ------------------------------------------------------------------ */
+/* ART diffuse-photon transfer (src/diffuse_photon/), one rank          */
+/* ------------------------------------------------------------------ */
+/* Layouts of src/diffuse_photon/diffuse_photon.h:77-160 with __HELIUM__
+ * on, __HELIUM_BB__ off and __USE_GPU__ on. */
+struct angle_info {                     /* 16 B, diffuse_photon.h:77-80 */
+    float xovr, yovr, zovr;
+    short base_id, corner_id;
+};
+struct step_func_factor {               /* 24 B, diffuse_photon.h:136-143 */
+    float HI[2], HeI[2], HeII[2];       /* [0] ionization, [1] heating */
+};
+struct radiation_mesh;                  /* library-owned device accumulators */
+struct omp_param;                       /* unused */
+struct host_diffuse_param {             /* 48 B, diffuse_photon.h:146-153 */
+    int corner_id_num[8];
+    struct step_func_factor *step_fact;
+    struct angle_info *angle;           /* [12*N_SIDE^2] */
+};
+struct cuda_diffuse_param {             /* 24 B, diffuse_photon.h:156-160 */
+    struct step_func_factor *step_fact;
+    struct angle_info *angle;
+    struct radiation_mesh *rmesh;
+};
+
+/* replaces init_hd_param / free_hd_param, src/diffuse_photon/init_hd_param.c:8-30:
+ * host arrays for N_ANG = 12*N_SIDE^2 directions (N_SIDE: see
+ * argot_hip_set_diffuse_nside) */
+void init_hd_param(struct host_diffuse_param *hd_param);
+void free_hd_param(struct host_diffuse_param *hd_param);
+
+/* replaces setup_diffuse_photon, src/diffuse_photon/setup_diffuse.c:8-14:
+ * HEALPix directions (set_angle_info, sort_angle_id) and the step-function
+ * factors (set_step_func_factor) */
+void setup_diffuse_photon(struct host_diffuse_param *hd_param, struct run_param *this_run);
+
+/* replaces init_diffuse_gpu, src/diffuse_photon/init_diffuse_gpu.cu:8-40,
+ * and send_diffuse_data, src/diffuse_photon/send_diffuse_data.cu:8-35:
+ * device buffers for the local mesh (even sizes, cubic cells), then the
+ * directions and factors */
+void init_diffuse_gpu(struct cuda_diffuse_param *cd_param, struct cuda_param *this_cuda,
+                      struct run_param *this_run);
+void send_diffuse_data(struct host_diffuse_param *hd_param, struct cuda_diffuse_param *cd_param,
+                       struct cuda_param *this_cuda);
+
+/* replaces calc_diffuse_photon_radiation (GPU variant),
+ * src/diffuse_photon/calc_diffuse_photon_radiation.c:38-120 and
+ * src/diffuse_photon/diffuse_prototype.h:30: absorption and recombination
+ * emissivity per cell, every direction's parallel rays, per-cell rates
+ * averaged over directions and added to mesh_dev[].prev_chem. */
+void calc_diffuse_photon_radiation(struct fluid_mesh *mesh, struct run_param *this_run,
+                                   struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                                   struct host_diffuse_param *hd_param,
+                                   struct cuda_diffuse_param *cd_param,
+                                   struct omp_param *this_omp);
+
+/* replaces free_diffuse_gpu, src/diffuse_photon/free_diffuse_gpu.cu:8-18 */
+void free_diffuse_gpu(struct cuda_diffuse_param *cd_param, struct cuda_param *this_cuda);
+
+/* ART controls (no reference counterpart): N_SIDE (reference: compile-time
+ * 8, run_param.h:159; takes effect at init_hd_param/setup/init), the
+ * __HEATCOOL__ step factors (default on, src/Makefile:187), directions
+ * traced per batch, Kahan-compensated sums over directions. */
+void argot_hip_set_diffuse_nside(int nside);
+void argot_hip_set_diffuse_heatcool(int on);
+void argot_hip_set_diffuse_batch(int ndir);
+void argot_hip_set_diffuse_kahan(int on);
+struct argot_diffuse_stats {
+    uint64_t ndir;           /* directions */
+    uint64_t ncross;         /* ray-cell crossings */
+    double ms_trace;         /* ray tracing kernels, HIP events */
+    double ms_gh;            /* per-direction rate kernels, HIP events */
+    double ms_total;         /* whole call, wall clock */
+};
+void argot_hip_diffuse_stats(struct argot_diffuse_stats *out);
+
+/* ------------------------------------------------------------------ */
 /* Library controls (no reference counterpart)                         */
 /* ------------------------------------------------------------------ */
 

@@ -180,6 +180,43 @@ uint64_t orc_ray_records(const orc_grid *g, const orc_fields *f,
 void orc_thin_kernel(int ngrid, const double *nu, const float *cs,
                      const double *spec, const double *tau_th, double *out4);
 
+/* ------------------------------------------------------------------ */
+/* ART diffuse-photon transfer (art.c), one rank                        */
+/* ------------------------------------------------------------------ */
+
+/* chealpix pix2ang_ring from the published ring-scheme formulas. */
+void orc_pix2ang_ring(long nside, long ipix, double *theta, double *phi);
+
+/* set_angle_info (+ sort_angle_id when sorted != 0): ovr[3*i] direction
+ * cosines (FP32), base_id, corner_id per direction; corner_num[8]. */
+void orc_art_angles(int nside, int sorted, float *ovr, short *base, short *corner,
+                    int *corner_num);
+
+/* set_step_func_factor: f = {HI[0], HI[1], HeI[0], HeI[1], HeII[0], HeII[1]};
+ * heatcool selects the __HEATCOOL__ 64-bin form. */
+void orc_art_step_factors(int heatcool, float f[6]);
+
+/* calc_rmesh_data: chem[7*c] = felec, fHI, fHII, fHeI, fHeII, fHeIII,
+ * prev_uene; kap/src [3*c] for the three bands. */
+void orc_art_rmesh(long ncell, const float *dens, const float *chem, double denstonh,
+                   double uenetok, double helium_fact, float *kap, float *src);
+
+/* One direction (zeroed accumulators, rays in group order): per-cell path
+ * length [cm] and band intensities I[3*c].  Returns the cell crossings. */
+long orc_art_direction(const orc_grid *g, const float ovr[3], int base, const float *kap,
+                       const float *src, float *length, float *I);
+
+/* calc_GH_tot: gh[6*c] += per-direction rates (GHI, HHI, GHeI, HHeI,
+ * GHeII, HHeII). */
+void orc_art_gh_tot(long ncell, const float *length, const float *I, const float *kap,
+                    const float *src, const float f[6], float *gh);
+
+/* The whole pass (sorted directions, calc_GH_sum): rates[6*c] in mesh
+ * order (GammaHI, HeatHI, GammaHeI, GammaHeII, HeatHeI, HeatHeII) +=.
+ * Returns the total cell crossings. */
+long orc_art_run(const orc_grid *g, int nside, int heatcool, const float *dens,
+                 const float *chem, double uenetok, float *rates);
+
 #ifdef __cplusplus
 }
 #endif

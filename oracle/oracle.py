@@ -91,6 +91,18 @@ def lib():
                                   f32p, i32p, u64p]
     L.orc_ray_records.restype = C.c_uint64
     L.orc_thin_kernel.argtypes = [C.c_int, f64p, f32p, f64p, f64p, f64p]
+    i16p = np.ctypeslib.ndpointer(dtype=np.int16, flags="C_CONTIGUOUS")
+    L.orc_pix2ang_ring.argtypes = [C.c_long, C.c_long, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]
+    L.orc_art_angles.argtypes = [C.c_int, C.c_int, f32p, i16p, i16p, i32p]
+    L.orc_art_step_factors.argtypes = [C.c_int, f32p]
+    L.orc_art_rmesh.argtypes = [C.c_long, f32p, f32p, C.c_double, C.c_double, C.c_double,
+                                f32p, f32p]
+    L.orc_art_direction.argtypes = [C.POINTER(Grid), f32p, C.c_int, f32p, f32p, f32p, f32p]
+    L.orc_art_direction.restype = C.c_long
+    L.orc_art_gh_tot.argtypes = [C.c_long, f32p, f32p, f32p, f32p, f32p, f32p]
+    L.orc_art_run.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, f32p, f32p, C.c_double, f32p]
+    L.orc_art_run.restype = C.c_long
     _lib = L
     return L
 
@@ -286,3 +298,71 @@ def thin_kernel(nu, cs, spec, tau_th):
                           np.ascontiguousarray(spec, np.float64),
                           np.ascontiguousarray(tau_th, np.float64), out)
     return out
+
+
+# ---------------------------------------------------------------------------
+# ART diffuse-photon transfer (art.c)
+
+def pix2ang_ring(nside, ipix):
+    th, ph = C.c_double(), C.c_double()
+    lib().orc_pix2ang_ring(nside, ipix, C.byref(th), C.byref(ph))
+    return th.value, ph.value
+
+
+def art_angles(nside, sorted_=True):
+    """(ovr [n,3] FP32, base_id [n], corner_id [n], corner_num [8])."""
+    n = 12 * nside * nside
+    ovr = np.zeros(3 * n, np.float32)
+    base = np.zeros(n, np.int16)
+    corner = np.zeros(n, np.int16)
+    cnum = np.zeros(8, np.int32)
+    lib().orc_art_angles(nside, 1 if sorted_ else 0, ovr, base, corner, cnum)
+    return ovr.reshape(n, 3), base, corner, cnum
+
+
+def art_step_factors(heatcool=True):
+    f = np.zeros(6, np.float32)
+    lib().orc_art_step_factors(1 if heatcool else 0, f)
+    return f
+
+
+def art_grid(n, lunit, denstonh, helium_fact, lo=0.0, hi=1.0):
+    h = float(np.float32(np.float32(hi - lo) / np.float32(n)))
+    g = Grid()
+    g.nx = g.ny = g.nz = n
+    g.xmin = g.ymin = g.zmin = lo
+    g.xmax = g.ymax = g.zmax = hi
+    g.dx = g.dy = g.dz = h
+    g.lunit, g.denstonh, g.helium_fact = lunit, denstonh, helium_fact
+    return g
+
+
+def art_rmesh(dens, chem, denstonh, uenetok, helium_fact):
+    """chem [ncell, 7] = felec, fHI, fHII, fHeI, fHeII, fHeIII, prev_uene."""
+    ncell = dens.size
+    kap = np.zeros(3 * ncell, np.float32)
+    src = np.zeros(3 * ncell, np.float32)
+    lib().orc_art_rmesh(ncell, np.ascontiguousarray(dens, np.float32).reshape(-1),
+                        np.ascontiguousarray(chem, np.float32).reshape(-1), denstonh, uenetok,
+                        helium_fact, kap, src)
+    return kap.reshape(ncell, 3), src.reshape(ncell, 3)
+
+
+def art_direction(g, ovr, base, kap, src):
+    ncell = g.nx * g.ny * g.nz
+    length = np.zeros(ncell, np.float32)
+    I = np.zeros(3 * ncell, np.float32)
+    nc = lib().orc_art_direction(C.byref(g), np.ascontiguousarray(ovr, np.float32), int(base),
+                                 np.ascontiguousarray(kap, np.float32).reshape(-1),
+                                 np.ascontiguousarray(src, np.float32).reshape(-1), length, I)
+    return length, I.reshape(ncell, 3), nc
+
+
+def art_run(g, nside, dens, chem, uenetok, heatcool=True):
+    """Rates [ncell, 6] in mesh order added to zeros, and the crossings."""
+    ncell = g.nx * g.ny * g.nz
+    rates = np.zeros(6 * ncell, np.float32)
+    nc = lib().orc_art_run(C.byref(g), nside, 1 if heatcool else 0,
+                           np.ascontiguousarray(dens, np.float32).reshape(-1),
+                           np.ascontiguousarray(chem, np.float32).reshape(-1), uenetok, rates)
+    return rates.reshape(ncell, 6), nc

@@ -63,3 +63,24 @@ def cell_centre(idx, n, lo=0.0, hi=1.0):
     double, rounded to float (setup_light_ray_tree.c:341-343)."""
     d = np.float64(np.float32(np.float32(hi - lo) / np.float32(n)))
     return np.float32(np.float64(np.float32(lo)) + (np.float64(np.float32(idx)) + 0.5) * d)
+
+
+def library_node_spectra(w, theta=0.7):
+    """Spectra [nnode, 256] of the library's own source tree (its node
+    labels, which are the `id` of kind-0 ray records)."""
+    import ctypes as C
+    from argot_amd._lib import lib
+    freq, _, _, _ = tables()
+    srcs = R.blackbody_sources(w.src_pos, w.src_rate, w.src_temp, freq)
+    spec = np.ascontiguousarray(spectra_of(srcs, w.nsrc))
+    pos = np.ascontiguousarray(w.src_pos, np.float32)
+    L = lib()
+    box = (C.c_float * 6)(0, 0, 0, 1, 1, 1)
+    h = L.argot_hip_tree_build(pos.ctypes.data, spec.ctypes.data, w.nsrc, spec.shape[1], box,
+                               theta)
+    try:
+        nn = L.argot_hip_tree_nnode(h)
+        return np.array([np.ctypeslib.as_array(L.argot_hip_tree_spec(h, u), shape=(spec.shape[1],))
+                         for u in range(nn)])
+    finally:
+        L.argot_hip_tree_free(h)

@@ -8,7 +8,7 @@ import pytest
 import mlp_ref as M
 from argot_amd import radiation as R
 from argot_amd import surrogate as S
-from helpers import oracle_problem, small_workload
+from helpers import library_node_spectra, oracle_problem, small_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -137,11 +137,13 @@ def test_step_serves_trusted_rays_within_validated_error(require_gpu):
     np.testing.assert_array_equal(rec0["tau"], rec["tau"])
     np.testing.assert_array_equal(rec0["rate"][~served], rec["rate"][~served])
     P = oracle_problem(w)
+    nodes = library_node_spectra(w)     # kind-0 ids are the library's node labels
+    spec = lambda i: nodes[rec["id"][i]] if rec["kind"][i] == 0 else P.ray_spec(1, rec["id"][i])
     n = w.n
     idx = np.nonzero(served)[0]
     rng = np.random.default_rng(0)
     idx = rng.choice(idx, min(len(idx), 3000), replace=False)
-    ref = np.array([P.ray_rate(rec["src"][i], P.ray_spec(int(rec["kind"][i]), int(rec["id"][i])),
+    ref = np.array([P.ray_rate(rec["src"][i], spec(i),
                                int(rec["target"][i]) // (n * n), (int(rec["target"][i]) // n) % n,
                                int(rec["target"][i]) % n, rec["tau"][i].astype(np.float64),
                                mode=1) for i in idx])
