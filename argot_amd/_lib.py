@@ -32,6 +32,7 @@ EXPORTS = [
     "send_diffuse_data", "calc_diffuse_photon_radiation", "free_diffuse_gpu",
     "argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
     "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
+    "argot_hip_set_diffuse_exp64",
 ]
 
 _lib = None
@@ -111,7 +112,8 @@ def lib():
     L.calc_diffuse_photon_radiation.argtypes = [C.c_void_p, rp, cm, cp, hd, cd, C.c_void_p]
     L.free_diffuse_gpu.argtypes = [cd, cp]
     for n in ("argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
-              "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan"):
+              "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan",
+              "argot_hip_set_diffuse_exp64"):
         getattr(L, n).argtypes = [C.c_int]
     L.argot_hip_diffuse_stats.argtypes = [P(abi.DiffuseStats)]
     _lib = L

@@ -51,10 +51,11 @@ def fill_art_mesh(w):
 class DiffuseStep:
     """One GPU, one rank: the diffuse transfer of a mesh."""
 
-    def __init__(self, w, nside=8, heatcool=True, batch=32, kahan=False):
+    def __init__(self, w, nside=8, heatcool=True, batch=32, kahan=False, exp64=False):
         L = lib()
         self.w = w
         self.nside, self.heatcool, self.batch, self.kahan = nside, heatcool, batch, kahan
+        self.exp64 = exp64
         self._settings()
         self.freq, self.cs = R.freq_and_csect()
         self.run = R.run_param_for(w, self.freq, self.cs)
@@ -84,6 +85,7 @@ class DiffuseStep:
         L.argot_hip_set_diffuse_heatcool(1 if self.heatcool else 0)
         L.argot_hip_set_diffuse_batch(self.batch)
         L.argot_hip_set_diffuse_kahan(1 if self.kahan else 0)
+        L.argot_hip_set_diffuse_exp64(1 if self.exp64 else 0)
 
     def angles(self):
         n = 12 * self.nside ** 2

@@ -107,7 +107,7 @@ __device__ inline double k06_B(double t) {
 __global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
                                  double denstonh, double uenetok, double helium_fact,
                                  float csHI, float csHeI, float csHeII,
-                                 float4* __restrict__ kap, float4* __restrict__ src) {
+                                 float4* __restrict__ med) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const unsigned char* m = mesh + c * kStride;
@@ -140,8 +140,8 @@ __global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int64_t
   s.z = (float)((k06_A(t) - k06_B(t)) * ne * nHeIII * kNuLeV * 4.0 * kEVtoErg / (4.0 * kPiArt) /
                 (double)k.z);
   s.w = 0.0f;
-  kap[c] = k;
-  src[c] = s;
+  med[2 * c] = k;
+  med[2 * c + 1] = s;
 }
 
 // ----------------------------------------------------------------------
@@ -153,21 +153,38 @@ struct TraceArgs {
   int ndir;
   int group;              // ray group 0..3 (set_boundary_ray's parity classes)
   int nray_max;           // rays per group and direction (largest face / 4)
-  const float4* kap;
-  const float4* src;
+  const float4* med;      // [2*ncell]: absorption (3 bands), source function (3 bands)
   float4* acc;            // [ndir][ncell] {length, I1, I2, I3}
   unsigned long long* ncross;
 };
 
-// One band of calc_intensity: the reference evaluates exp and expm1 in
-// double on the float optical depth and rounds to float.
+// One band of calc_intensity.  F64: as the reference, exp and expm1 in
+// double on the float optical depth, rounded to float.  Otherwise FP32:
+// exp(-tau) from the hardware exponential (a few ulp), 1 - exp(-tau) from
+// its Taylor series below 0.25 (truncation < 2e-9 relative) so that thin
+// cells keep full relative precision.
+template <bool F64>
 __device__ __forceinline__ void band(float tau, float src, float& Icell, float& Iin) {
-  const float etau = (float)exp(-(double)tau);
-  const float etaum1 = (float)(-expm1(-(double)tau));
+  float etau, etaum1;
+  if (F64) {
+    etau = (float)exp(-(double)tau);
+    etaum1 = (float)(-expm1(-(double)tau));
+  } else {
+    etau = __expf(-tau);
+    float p = __builtin_fmaf(tau, -1.0f / 5040, 1.0f / 720);
+    p = __builtin_fmaf(tau, p, -1.0f / 120);
+    p = __builtin_fmaf(tau, p, 1.0f / 24);
+    p = __builtin_fmaf(tau, p, -1.0f / 6);
+    p = __builtin_fmaf(tau, p, 1.0f / 2);
+    p = __builtin_fmaf(tau, p, -1.0f);
+    const float series = -tau * p;   // tau - tau^2/2 + ... + tau^7/5040
+    etaum1 = tau < 0.25f ? series : 1.0f - etau;
+  }
   Icell += Iin * etaum1;
   Iin = Iin * etau + src * etaum1;
 }
 
+template <bool F64>
 __global__ __launch_bounds__(256) void art_trace_kernel(TraceArgs a) {
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int d = (int)(gid / a.nray_max);
@@ -190,9 +207,11 @@ __global__ __launch_bounds__(256) void art_trace_kernel(TraceArgs a) {
       k = 2 * (r % (g.nz >> 1)) + db;
       i = base == 2 ? 0 : g.nx - 1;
     } else {
+      // neighbouring threads take rays two cells apart in z (the fastest
+      // axis of the mesh), so that their accesses share cache lines
       nface = g.nz * g.nx / 4;
-      k = 2 * (r / (g.nx >> 1)) + da;
-      i = 2 * (r % (g.nx >> 1)) + db;
+      i = 2 * (r / (g.nz >> 1)) + db;
+      k = 2 * (r % (g.nz >> 1)) + da;
       j = base == 4 ? 0 : g.ny - 1;
     }
     if (r < nface) {
@@ -221,12 +240,24 @@ __global__ __launch_bounds__(256) void art_trace_kernel(TraceArgs a) {
         return ix >= g.nx || iy >= g.ny || iz >= g.nz || ix < 0 || iy < 0 || iz < 0;
       };
       float I0 = 0.0f, I1 = 0.0f, I2 = 0.0f;
-      float4* acc = a.acc + (int64_t)d * g.ncell;
+      float4* __restrict__ acc = a.acc + (int64_t)d * g.ncell;
+      const float4* __restrict__ med = a.med;
+      auto cell = [&]() {
+        return (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
+      };
       bool live = !out();
+      int64_t pos = 0;
+      float4 kp{}, sp{}, ac{};
+      if (live) {
+        pos = cell();
+        kp = med[2 * pos];
+        sp = med[2 * pos + 1];
+        ac = acc[pos];
+      }
+      // Each pass: the crossing's geometry and the next cell (with the
+      // side-face wrap) first, the next cell's loads issued, then the
+      // current cell's intensities (the geometry never reads the medium).
       while (live) {
-        const int64_t pos = (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
-        const float4 kp = a.kap[pos];
-        const float4 sp = a.src[pos];
         const float ddx = g.dx * (float)(xs + ix) - (x - g.xmin);
         const float ddy = g.dy * (float)(ys + iy) - (y - g.ymin);
         const float ddz = g.dz * (float)(zs + iz) - (z - g.zmin);
@@ -244,35 +275,56 @@ __global__ __launch_bounds__(256) void art_trace_kernel(TraceArgs a) {
         if (rdy == rmin) iy += yn;
         if (rdz == rmin) iz += zn;
         rmin = (float)((double)rmin * g.lunit);
-        float4 ac = acc[pos];
+        // next cell: far face ends the ray; a side face wraps it to the
+        // opposite face, whose intensity restarts at zero after this
+        // cell's update (ray_cycle + reset_I on one rank)
+        bool wrapped = false, cont = true;
+        if (out()) {
+          const bool far = base <= 1 ? (iz >= g.nz || iz < 0)
+                                     : (base <= 3 ? (ix >= g.nx || ix < 0) : (iy >= g.ny || iy < 0));
+          if (far) {
+            cont = false;
+          } else {
+            auto wrap_x = [&]() {
+              if (ix >= g.nx) { ix = 0; x = g.xmin; wrapped = true; }
+              else if (ix < 0) { ix = g.nx - 1; x = g.xmax; wrapped = true; }
+            };
+            auto wrap_y = [&]() {
+              if (iy >= g.ny) { iy = 0; y = g.ymin; wrapped = true; }
+              else if (iy < 0) { iy = g.ny - 1; y = g.ymax; wrapped = true; }
+            };
+            auto wrap_z = [&]() {
+              if (iz >= g.nz) { iz = 0; z = g.zmin; wrapped = true; }
+              else if (iz < 0) { iz = g.nz - 1; z = g.zmax; wrapped = true; }
+            };
+            if (base <= 1) { wrap_x(); wrap_y(); }
+            else if (base <= 3) { wrap_y(); wrap_z(); }
+            else { wrap_z(); wrap_x(); }
+            cont = !out();
+          }
+        }
+        int64_t npos = pos;
+        float4 nkp = kp, nsp = sp, nac = ac;
+        if (cont) {
+          // another ray of this group never touches these cells, and this
+          // ray never returns to one: the early read of acc is safe
+          npos = cell();
+          nkp = med[2 * npos];
+          nsp = med[2 * npos + 1];
+          nac = acc[npos];
+        }
         ac.x += rmin;
-        band(kp.x * rmin, sp.x, ac.y, I0);
-        band(kp.y * rmin, sp.y, ac.z, I1);
-        band(kp.z * rmin, sp.z, ac.w, I2);
+        band<F64>(kp.x * rmin, sp.x, ac.y, I0);
+        band<F64>(kp.y * rmin, sp.y, ac.z, I1);
+        band<F64>(kp.z * rmin, sp.z, ac.w, I2);
         acc[pos] = ac;
         nc++;
-        if (!out()) continue;
-        // far face: the ray ends; side faces: re-enter at the opposite
-        // face with zero intensity (ray_cycle + reset_I on one rank)
-        const bool far = base <= 1 ? (iz >= g.nz || iz < 0)
-                                   : (base <= 3 ? (ix >= g.nx || ix < 0) : (iy >= g.ny || iy < 0));
-        if (far) break;
-        auto wrap_x = [&]() {
-          if (ix >= g.nx) { ix = 0; x = g.xmin; I0 = I1 = I2 = 0.0f; }
-          else if (ix < 0) { ix = g.nx - 1; x = g.xmax; I0 = I1 = I2 = 0.0f; }
-        };
-        auto wrap_y = [&]() {
-          if (iy >= g.ny) { iy = 0; y = g.ymin; I0 = I1 = I2 = 0.0f; }
-          else if (iy < 0) { iy = g.ny - 1; y = g.ymax; I0 = I1 = I2 = 0.0f; }
-        };
-        auto wrap_z = [&]() {
-          if (iz >= g.nz) { iz = 0; z = g.zmin; I0 = I1 = I2 = 0.0f; }
-          else if (iz < 0) { iz = g.nz - 1; z = g.zmax; I0 = I1 = I2 = 0.0f; }
-        };
-        if (base <= 1) { wrap_x(); wrap_y(); }
-        else if (base <= 3) { wrap_y(); wrap_z(); }
-        else { wrap_z(); wrap_x(); }
-        live = !out();
+        if (wrapped) I0 = I1 = I2 = 0.0f;
+        pos = npos;
+        kp = nkp;
+        sp = nsp;
+        ac = nac;
+        live = cont;
       }
     }
   }
@@ -293,12 +345,12 @@ __global__ void art_fill_kernel(float4* __restrict__ p, int64_t n) {
 // across directions.
 template <bool KAHAN>
 __global__ void art_gh_kernel(const float4* __restrict__ acc, int ndir, int64_t ncell,
-                              const float4* __restrict__ kap, const float4* __restrict__ src,
+                              const float4* __restrict__ med,
                               float f0, float f1, float f2, float f3, float f4, float f5,
                               float* __restrict__ gh, float* __restrict__ comp) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const float4 ka = kap[c], so = src[c];
+  const float4 ka = med[2 * c], so = med[2 * c + 1];
   float s[6], e[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
@@ -389,6 +441,7 @@ int g_nside = 8;             // N_SIDE (run_param.h:159)
 int g_heatcool = 1;          // __HEATCOOL__ (src/Makefile:187)
 int g_batch = 32;            // directions per batch
 int g_kahan = 0;
+int g_exp64 = 0;             // 1: exp/expm1 in FP64 as the reference
 argot_diffuse_stats g_dstats{};
 
 struct ArtCtx {
@@ -398,7 +451,7 @@ struct ArtCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ArtDir* d_dirs = nullptr;
-  float4 *kap = nullptr, *src = nullptr, *acc = nullptr;
+  float4 *med = nullptr, *acc = nullptr;   // med: [2*ncell] absorption, source
   float *gh = nullptr, *comp = nullptr;
   unsigned long long* ncross = nullptr;
   int64_t ncell = 0;
@@ -558,6 +611,7 @@ void argot_hip_set_diffuse_nside(int nside) { g_nside = nside > 0 ? nside : 8; }
 void argot_hip_set_diffuse_heatcool(int on) { g_heatcool = on ? 1 : 0; }
 void argot_hip_set_diffuse_batch(int ndir) { g_batch = ndir > 0 ? ndir : 32; }
 void argot_hip_set_diffuse_kahan(int on) { g_kahan = on ? 1 : 0; }
+void argot_hip_set_diffuse_exp64(int on) { g_exp64 = on ? 1 : 0; }
 void argot_hip_diffuse_stats(struct argot_diffuse_stats* out) {
   if (out) *out = g_dstats;
 }
@@ -602,17 +656,16 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
     A_OK(hipStreamCreate(&a_.stream));
     for (auto& e : a_.ev) A_OK(hipEventCreate(&e));
   }
-  for (void* p : {(void*)a_.d_dirs, (void*)a_.kap, (void*)a_.src, (void*)a_.acc, (void*)a_.gh,
+  for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
                   (void*)a_.comp, (void*)a_.ncross})
     if (p) A_OK(hipFree(p));
   a_.d_dirs = nullptr;
-  a_.kap = a_.src = a_.acc = nullptr;
+  a_.med = a_.acc = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
   A_OK(hipMalloc(&a_.d_dirs, sizeof(ArtDir) * nang));
-  A_OK(hipMalloc(&a_.kap, sizeof(float4) * ncell));
-  A_OK(hipMalloc(&a_.src, sizeof(float4) * ncell));
+  A_OK(hipMalloc(&a_.med, sizeof(float4) * 2 * ncell));
   A_OK(hipMalloc(&a_.acc, sizeof(float4) * ncell * batch));
   A_OK(hipMalloc(&a_.gh, sizeof(float) * 6 * ncell));
   A_OK(hipMalloc(&a_.comp, sizeof(float) * 6 * ncell));
@@ -693,7 +746,7 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
   art_rmesh_kernel<<<nb, 256, 0, st>>>(
       dmesh, nc, this_run->denstonh, this_run->uenetok, argot_hip_internal_helium_fact(),
       (float)csect_HI(1.0 + 1.0e-7), (float)csect_HeI(1.8125 + 1.0e-7),
-      (float)csect_HeII(4.0 + 1.0e-7), a_.kap, a_.src);
+      (float)csect_HeII(4.0 + 1.0e-7), a_.med);
   A_OK(hipGetLastError());
   A_OK(hipMemsetAsync(a_.gh, 0, sizeof(float) * 6 * nc, st));
   A_OK(hipMemsetAsync(a_.comp, 0, sizeof(float) * 6 * nc, st));
@@ -711,22 +764,24 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
     ta.dirs = a_.d_dirs + d0;
     ta.ndir = nd;
     ta.nray_max = nray_max;
-    ta.kap = a_.kap;
-    ta.src = a_.src;
+    ta.med = a_.med;
     ta.acc = a_.acc;
     ta.ncross = a_.ncross;
     const int64_t nthread = (int64_t)nd * nray_max;
     for (int grp = 0; grp < 4; grp++) {
       ta.group = grp;
-      art_trace_kernel<<<(unsigned)((nthread + 255) / 256), 256, 0, st>>>(ta);
+      if (g_exp64)
+        art_trace_kernel<true><<<(unsigned)((nthread + 255) / 256), 256, 0, st>>>(ta);
+      else
+        art_trace_kernel<false><<<(unsigned)((nthread + 255) / 256), 256, 0, st>>>(ta);
     }
     A_OK(hipEventRecord(a_.ev[2], st));
     const float* f = a_.fact;
     if (g_kahan)
-      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.kap, a_.src, f[0], f[1], f[2],
+      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.med, f[0], f[1], f[2],
                                               f[3], f[4], f[5], a_.gh, a_.comp);
     else
-      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.kap, a_.src, f[0], f[1],
+      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.med, f[0], f[1],
                                                f[2], f[3], f[4], f[5], a_.gh, a_.comp);
     A_OK(hipGetLastError());
     A_OK(hipEventRecord(a_.ev[3], st));
@@ -753,11 +808,11 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
 
 void free_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cuda) {
   (void)this_cuda;
-  for (void* p : {(void*)a_.d_dirs, (void*)a_.kap, (void*)a_.src, (void*)a_.acc, (void*)a_.gh,
+  for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
                   (void*)a_.comp, (void*)a_.ncross})
     if (p) (void)hipFree(p);
   a_.d_dirs = nullptr;
-  a_.kap = a_.src = a_.acc = nullptr;
+  a_.med = a_.acc = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();

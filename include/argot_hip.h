@@ -316,6 +316,10 @@ void argot_hip_set_diffuse_nside(int nside);
 void argot_hip_set_diffuse_heatcool(int on);
 void argot_hip_set_diffuse_batch(int ndir);
 void argot_hip_set_diffuse_kahan(int on);
+/* 1: exp and expm1 of the band optical depths in FP64 rounded to FP32, as
+ * the reference's calc_intensity; 0 (default): FP32 exponential with a
+ * Taylor series for 1 - exp(-tau) below 0.25. */
+void argot_hip_set_diffuse_exp64(int on);
 struct argot_diffuse_stats {
     uint64_t ndir;           /* directions */
     uint64_t ncross;         /* ray-cell crossings */

@@ -23,11 +23,12 @@ def rel(a, b):
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
 
 
-@pytest.mark.parametrize("n,nside,heatcool", [(16, 2, True), (16, 8, True), (32, 4, False)])
-def test_diffuse_pass_matches_oracle(require_gpu, n, nside, heatcool):
+@pytest.mark.parametrize("n,nside,heatcool,exp64", [(16, 2, True, True), (16, 8, True, True),
+                                                    (16, 8, True, False), (32, 4, False, False)])
+def test_diffuse_pass_matches_oracle(require_gpu, n, nside, heatcool, exp64):
     w = ic.art128(seed=11, n=n)
-    print(f"art n={n} nside={nside} heatcool={heatcool}", flush=True)
-    st = A.DiffuseStep(w, nside=nside, heatcool=heatcool, batch=16)
+    print(f"art n={n} nside={nside} heatcool={heatcool} exp64={exp64}", flush=True)
+    st = A.DiffuseStep(w, nside=nside, heatcool=heatcool, batch=16, exp64=exp64)
     stats = st.step()
     got = st.rates()
     ref, ncross = oracle_rates(w, nside, heatcool)
