@@ -32,7 +32,8 @@ EXPORTS = [
     "send_diffuse_data", "calc_diffuse_photon_radiation", "free_diffuse_gpu",
     "argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
     "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
-    "argot_hip_set_diffuse_exp64",
+    "argot_hip_set_diffuse_exp64", "argot_hip_set_tile", "argot_hip_set_graph",
+    "argot_hip_graph_captures", "argot_hip_autotune_step",
 ]
 
 _lib = None
@@ -116,6 +117,10 @@ def lib():
               "argot_hip_set_diffuse_exp64"):
         getattr(L, n).argtypes = [C.c_int]
     L.argot_hip_diffuse_stats.argtypes = [P(abi.DiffuseStats)]
+    L.argot_hip_set_tile.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.argot_hip_set_graph.argtypes = [C.c_int]
+    L.argot_hip_graph_captures.restype = C.c_int64
+    L.argot_hip_autotune_step.argtypes = [C.c_void_p, C.c_void_p, rp, cm, cp, P(abi.TuneReport)]
     _lib = L
     return L
 

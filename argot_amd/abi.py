@@ -194,3 +194,14 @@ class CudaDiffuseParam(C.Structure):
 class DiffuseStats(C.Structure):
     _fields_ = [("ndir", C.c_uint64), ("ncross", C.c_uint64), ("ms_trace", C.c_double),
                 ("ms_gh", C.c_double), ("ms_total", C.c_double)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("tile", C.c_int * 3), ("blocks_per_cu", C.c_int), ("ms_kernel", C.c_double)]
+
+
+TUNE_MAX = 32
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n", C.c_int), ("best", C.c_int), ("entry", TuneEntry * TUNE_MAX)]

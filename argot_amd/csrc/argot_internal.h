@@ -17,7 +17,7 @@ namespace argot {
 
 constexpr int kMaxSegments = 5;   // NSEG_PER_RAY (run_param.h:60)
 constexpr int kNgridMax = 256;    // NGRID_NU (run_param.h:78)
-constexpr int kTile = 4;          // 4x4x4 target cells per wavefront
+constexpr int kTileBits = 6;      // 64 target cells (one per lane) per wavefront tile
 constexpr int kNumQueues = 8;     // one tile queue per XCD
 
 // Mesh geometry and physical units as the kernels use them.

@@ -17,6 +17,7 @@ struct FusedArgs {
   RayRecord* rec;              // capture mode only
   int64_t rec_cap;
   SurrDev surr;                // rate surrogate (surrogate launches only)
+  int tile_bits[3];            // target tile of a wavefront: 2^bx x 2^by x 2^bz = 64 cells
 };
 
 hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fact, float4* nf,

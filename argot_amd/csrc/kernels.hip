@@ -318,8 +318,9 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   const TreeDev& tr = a.tree;
   const RateTables& rt = a.rates;
   const int lane = threadIdx.x;
-  const int ntx = (g.nx + kTile - 1) / kTile, nty = (g.ny + kTile - 1) / kTile,
-            ntz = (g.nz + kTile - 1) / kTile;
+  const int bx = a.tile_bits[0], by = a.tile_bits[1], bz = a.tile_bits[2];
+  const int TX = 1 << bx, TY = 1 << by, TZ = 1 << bz;
+  const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, ntz = (g.nz + TZ - 1) / TZ;
   const int64_t ntile = (int64_t)ntx * nty * ntz;
   int q = blockIdx.x % kNumQueues, tried = 0;
   unsigned long long my_ray = 0, my_cross = 0, my_fb = 0, my_surr = 0;
@@ -335,9 +336,9 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     const int tz = (int)(t % ntz);
     const int ty = (int)((t / ntz) % nty);
     const int tx = (int)(t / ((int64_t)ntz * nty));
-    const int ix = tx * kTile + ((lane >> 4) & 3);
-    const int iy = ty * kTile + ((lane >> 2) & 3);
-    const int iz = tz * kTile + (lane & 3);
+    const int ix = tx * TX + ((lane >> (by + bz)) & (TX - 1));
+    const int iy = ty * TY + ((lane >> bz) & (TY - 1));
+    const int iz = tz * TZ + (lane & (TZ - 1));
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
     const int64_t cell = (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
     const Vec3f tgt{cell_centre(g.gxmin, ix + g.ox, g.dx), cell_centre(g.gymin, iy + g.oy, g.dy),

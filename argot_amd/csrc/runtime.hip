@@ -42,6 +42,17 @@ int g_tau_mode = 0;
 int g_rate_mode = 0;
 float g_theta = 0.7f;                  // ARGOT_THETA_CRIT, run_param.h:80
 int g_blocks_per_cu = 16;
+int g_tile_bits[3] = {2, 2, 2};        // 4x4x4 targets per wavefront
+int g_use_graph = 1;
+
+// Everything a captured step graph depends on.
+struct GraphKey {
+  FusedArgs args;
+  int nblock, kahan, surr;
+  int64_t ncell;
+  double helium_fact;
+  const float4* nf;
+};
 RayRecord* g_capture_host = nullptr;
 int64_t g_capture_cap = 0;
 
@@ -67,7 +78,7 @@ struct DevBuf {
 struct Ctx {
   bool ready = false;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
   int ncu = 256;
   int nx = 0, ny = 0, nz = 0;
   int64_t ncell = 0;
@@ -91,12 +102,17 @@ struct Ctx {
   DevBuf<float> rate_tab;
   HostTree tree;
   argot_step_stats stats{};
+  // step graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  GraphKey graph_key{};
+  int64_t stats_graph_captures = 0;
 };
 
 Ctx g;
 
 GridDev make_grid(const run_param* r) {
-  GridDev d;
+  GridDev d{};   // zeroed padding: launch arguments are compared as bytes
   d.nx = r->nmesh_x_local;
   d.ny = r->nmesh_y_local;
   d.nz = r->nmesh_z_local;
@@ -125,7 +141,7 @@ GridDev make_grid(const run_param* r) {
 }
 
 RateTables rate_tables() {
-  RateTables t;
+  RateTables t{};
   t.ngrid = g.ngrid;
   t.nu = g.nu.p;
   t.cs = g.cs.p;
@@ -247,6 +263,23 @@ void argot_hip_set_theta(float theta) { g_theta = theta; }
 // Persistent grid size of the fused kernel, in wavefronts per CU.
 void argot_hip_set_blocks_per_cu(int n) { g_blocks_per_cu = n > 0 ? n : 16; }
 
+void argot_hip_set_tile(int tx, int ty, int tz) {
+  t_err = ARGOT_OK;
+  auto log2i = [](int v) {
+    int b = 0;
+    while ((1 << b) < v) b++;
+    return (v > 0 && (1 << b) == v) ? b : -1;
+  };
+  const int b[3] = {log2i(tx), log2i(ty), log2i(tz)};
+  if (b[0] < 0 || b[1] < 0 || b[2] < 0 || b[0] + b[1] + b[2] != kTileBits)
+    return set_err(ARGOT_ERR_ARG);
+  for (int d = 0; d < 3; d++) g_tile_bits[d] = b[d];
+}
+
+void argot_hip_set_graph(int on) { g_use_graph = on ? 1 : 0; }
+
+int64_t argot_hip_graph_captures(void) { return g.stats_graph_captures; }
+
 // Test support: the next step_radiation_tree records every ray it forms
 // (struct RayRecord, 120 B) into host_records[0..cap).
 void argot_hip_set_capture(void* host_records, int64_t cap) {
@@ -287,6 +320,7 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   if (!g.stream) HIP_OK(hipStreamCreate(&g.stream));
   if (!g.ev0) HIP_OK(hipEventCreate(&g.ev0));
   if (!g.ev1) HIP_OK(hipEventCreate(&g.ev1));
+  if (!g.ev_mid) HIP_OK(hipEventCreate(&g.ev_mid));
   this_cuda->strm[0] = g.stream;
   this_cuda->diffuse_strm[0] = g.stream;
   g.mesh = reinterpret_cast<unsigned char*>(cuda_mem->mesh_dev);
@@ -516,14 +550,11 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
                         hipMemcpyHostToDevice, g.stream));
   const double t_built = now_ms();
 
-  // Per-cell FP32 densities, rates zeroed (zero_out_photoion_rate_cuda).
-  pack_fields();
-  if (t_err) return;
-  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
-  HIP_OK(launch_reset_counters(g.ctr.p, g.stream));
+  HIP_OK(g.nf.reserve(g.ncell));
 
-  FusedArgs a;
+  FusedArgs a{};   // zero-initialised: the graph cache compares its bytes
   a.grid = make_grid(this_run);
+  for (int d = 0; d < 3; d++) a.tile_bits[d] = g_tile_bits[d];
   a.tree.nnode = nsrc > 0 ? nn : 0;
   a.tree.node_cm = g.cm.p;
   a.tree.link = g.link.p;
@@ -555,14 +586,59 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   a.rec = capture ? g.rec.p : nullptr;
   a.rec_cap = capture ? g_capture_cap : 0;
   const int nblock = std::max(1, g.ncu * g_blocks_per_cu);
-  HIP_OK(hipEventRecord(g.ev0, g.stream));
-  HIP_OK(launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream));
-  HIP_OK(hipEventRecord(g.ev1, g.stream));
+  // Device work of the step: per-cell FP32 densities, rates zeroed
+  // (zero_out_photoion_rate_cuda), counters reset, the fused persistent
+  // kernel.  All kernels (no memset nodes), so the sequence can live in a
+  // HIP graph; the graph is re-captured when any launch argument changes.
+  auto enqueue = [&]() -> hipError_t {
+    hipError_t e = launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream);
+    if (e == hipSuccess) e = launch_zero_rates(g.mesh, g.ncell, g.stream);
+    if (e == hipSuccess) e = launch_reset_counters(g.ctr.p, g.stream);
+    if (e == hipSuccess) e = hipEventRecord(g.ev_mid, g.stream);   // fused kernel alone
+    if (e == hipSuccess) e = launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream);
+    return e;
+  };
+  const bool use_graph = g_use_graph && !capture;
+  if (use_graph) {
+    GraphKey key{};
+    key.args = a;
+    key.nblock = nblock;
+    key.kahan = g_tau_mode;
+    key.surr = surr;
+    key.ncell = g.ncell;
+    key.helium_fact = g_helium_fact;
+    key.nf = g.nf.p;
+    if (!g.graph_exec || memcmp(&key, &g.graph_key, sizeof key) != 0) {
+      if (g.graph_exec) HIP_OK(hipGraphExecDestroy(g.graph_exec));
+      if (g.graph) HIP_OK(hipGraphDestroy(g.graph));
+      g.graph_exec = nullptr;
+      g.graph = nullptr;
+      HIP_OK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = enqueue();
+      hipGraph_t graph = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(g.stream, &graph);
+      HIP_OK(e);
+      HIP_OK(e2);
+      g.graph = graph;
+      HIP_OK(hipGraphInstantiate(&g.graph_exec, g.graph, nullptr, nullptr, 0));
+      g.graph_key = key;
+      g.stats_graph_captures++;
+    }
+    HIP_OK(hipEventRecord(g.ev0, g.stream));
+    HIP_OK(hipGraphLaunch(g.graph_exec, g.stream));
+    HIP_OK(hipEventRecord(g.ev1, g.stream));
+  } else {
+    HIP_OK(hipEventRecord(g.ev0, g.stream));
+    HIP_OK(enqueue());
+    HIP_OK(hipEventRecord(g.ev1, g.stream));
+  }
   StepCounters hc;
   HIP_OK(hipMemcpyAsync(&hc, g.ctr.p, sizeof(hc), hipMemcpyDeviceToHost, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
   float ms = 0.0f;
-  HIP_OK(hipEventElapsedTime(&ms, g.ev0, g.ev1));
+  HIP_OK(hipEventElapsedTime(&ms, g.ev_mid, g.ev1));
+  float ms_all = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms_all, g.ev0, g.ev1));
   if (capture) {
     const int64_t n = std::min<int64_t>((int64_t)hc.nrecord, g_capture_cap);
     HIP_OK(hipMemcpy(g_capture_host, g.rec.p, sizeof(RayRecord) * n, hipMemcpyDeviceToHost));
@@ -581,6 +657,50 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
 // streaming kernel; returns the best GB/s (read + write bytes / time).
+// Autotuner: runs the step on the caller's inputs for each target-tile shape
+// (the 64 targets of a wavefront share every tree decision, so the shape
+// sets how coherent the wavefront's rays are) and persistent-grid size,
+// keeps the fastest fused kernel, and leaves it selected.
+void argot_hip_autotune_step(struct fluid_mesh* mesh, struct radiation_src* src,
+                             struct run_param* this_run, struct cuda_mem_space* cuda_mem,
+                             struct cuda_param* this_cuda, struct argot_tune_report* rep) {
+  static const int kTiles[][3] = {{4, 4, 4}, {2, 4, 8}, {4, 2, 8}, {1, 8, 8}, {8, 8, 1},
+                                  {2, 2, 16}, {8, 4, 2}, {4, 8, 2}};
+  static const int kBlocks[] = {8, 16, 32};
+  t_err = ARGOT_OK;
+  if (!rep) return set_err(ARGOT_ERR_ARG);
+  memset(rep, 0, sizeof(*rep));
+  const int old_bits[3] = {g_tile_bits[0], g_tile_bits[1], g_tile_bits[2]};
+  const int old_bpc = g_blocks_per_cu;
+  double best = 1e300;
+  int n = 0;
+  for (const auto& tl : kTiles)
+    for (int bpc : kBlocks) {
+      if (n >= ARGOT_TUNE_MAX) break;
+      argot_hip_set_tile(tl[0], tl[1], tl[2]);
+      g_blocks_per_cu = bpc;
+      step_radiation_tree(mesh, src, this_run, nullptr, cuda_mem, this_cuda, 0.0f);
+      if (t_err) {
+        for (int d = 0; d < 3; d++) g_tile_bits[d] = old_bits[d];
+        g_blocks_per_cu = old_bpc;
+        return;
+      }
+      argot_tune_entry& e = rep->entry[n];
+      for (int d = 0; d < 3; d++) e.tile[d] = tl[d];
+      e.blocks_per_cu = bpc;
+      e.ms_kernel = g.stats.ms_kernel;
+      if (e.ms_kernel < best) {
+        best = e.ms_kernel;
+        rep->best = n;
+      }
+      n++;
+    }
+  rep->n = n;
+  const argot_tune_entry& b = rep->entry[rep->best];
+  argot_hip_set_tile(b.tile[0], b.tile[1], b.tile[2]);
+  g_blocks_per_cu = b.blocks_per_cu;
+}
+
 double argot_hip_copy_probe(int64_t nbytes, int reps) {
   t_err = ARGOT_OK;
   if (nbytes < 16 || reps < 1) {

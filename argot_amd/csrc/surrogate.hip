@@ -516,7 +516,7 @@ hipError_t ensure_params() {
 }
 
 MlpDev net_dev() {
-  MlpDev d;
+  MlpDev d{};
   d.Pq = s_.Pq;
   d.Wb = s_.Wb;
   d.xc = s_.xc;
@@ -597,7 +597,7 @@ bool surrogate_ready() { return s_.have_net && s_.trained; }
 bool surrogate_any_trusted() { return surrogate_ready() && s_.ntrusted > 0; }
 
 SurrDev surrogate_dev() {
-  SurrDev d;
+  SurrDev d{};
   d.net = net_dev();
   d.trust = s_.trust;
   d.trust_n = s_.trust_n;

@@ -125,6 +125,22 @@ class PointSourceStep:
         check("step_radiation_tree")
         return self.stats()
 
+    def autotune(self):
+        """Time every tile shape / grid size on this step's inputs, keep the
+        fastest (argot_hip_autotune_step); returns the candidates and the
+        selected index."""
+        L = lib()
+        L.argot_hip_set_helium_fact(self.w.helium_fact)
+        L.argot_hip_set_tau_mode(self.tau_mode)
+        L.argot_hip_set_rate_mode(self.rate_mode)
+        rep = abi.TuneReport()
+        L.argot_hip_autotune_step(self.mesh, self.src, C.byref(self.run), C.byref(self.cmem),
+                                  C.byref(self.cpar), C.byref(rep))
+        check("argot_hip_autotune_step")
+        rows = [{"tile": tuple(rep.entry[i].tile), "blocks_per_cu": rep.entry[i].blocks_per_cu,
+                 "ms_kernel": rep.entry[i].ms_kernel} for i in range(rep.n)]
+        return rows, rep.best
+
     def stats(self):
         st = abi.StepStats()
         lib().argot_hip_step_stats(C.byref(st))

@@ -449,6 +449,34 @@ void argot_hip_set_theta(float theta);
 /* Persistent grid of the fused kernel, in wavefronts per compute unit. */
 void argot_hip_set_blocks_per_cu(int n);
 
+/* Target tile of one wavefront in the fused kernel: tx*ty*tz = 64 cells,
+ * each a power of two (default 4x4x4). */
+void argot_hip_set_tile(int tx, int ty, int tz);
+
+/* 1 (default): step_radiation_tree replays its device work (field packing,
+ * rate zeroing, counter reset, fused kernel) as one HIP graph, captured
+ * again only when a launch argument changes; 0: plain stream launches. */
+void argot_hip_set_graph(int on);
+/* Graph captures so far (tests). */
+int64_t argot_hip_graph_captures(void);
+
+/* Autotuner: times the step on the caller's inputs for each tile shape and
+ * persistent-grid size, keeps the fastest, and reports every candidate. */
+#define ARGOT_TUNE_MAX 32
+struct argot_tune_entry {
+    int tile[3];
+    int blocks_per_cu;
+    double ms_kernel;
+};
+struct argot_tune_report {
+    int n;                   /* candidates run */
+    int best;                /* index of the selected one */
+    struct argot_tune_entry entry[ARGOT_TUNE_MAX];
+};
+void argot_hip_autotune_step(struct fluid_mesh *mesh, struct radiation_src *src,
+                             struct run_param *this_run, struct cuda_mem_space *cuda_mem,
+                             struct cuda_param *this_cuda, struct argot_tune_report *rep);
+
 /* Measured device copy bandwidth: best of reps copies of nbytes with a
  * plain streaming kernel, in GB/s of bytes read plus written. */
 double argot_hip_copy_probe(int64_t nbytes, int reps);
