@@ -33,7 +33,7 @@ EXPORTS = [
     "argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
     "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
     "argot_hip_set_diffuse_exp64", "argot_hip_set_tile", "argot_hip_set_graph",
-    "argot_hip_graph_captures", "argot_hip_autotune_step",
+    "argot_hip_graph_captures", "argot_hip_autotune_step", "argot_hip_last_error_detail",
 ]
 
 _lib = None
@@ -71,6 +71,7 @@ def lib():
     L.argot_hip_last_error.restype = C.c_int
     L.argot_hip_error_string.argtypes = [C.c_int]
     L.argot_hip_error_string.restype = C.c_char_p
+    L.argot_hip_last_error_detail.restype = C.c_char_p
     L.argot_hip_set_helium_fact.argtypes = [C.c_double]
     L.argot_hip_set_tau_mode.argtypes = [C.c_int]
     L.argot_hip_set_rate_mode.argtypes = [C.c_int]
@@ -130,4 +131,5 @@ def check(what):
     e = lib().argot_hip_last_error()
     if e:
         msg = lib().argot_hip_error_string(e).decode()
-        raise ArgotError(f"{what} failed: {msg} (code {e})")
+        detail = lib().argot_hip_last_error_detail().decode()
+        raise ArgotError(f"{what} failed: {msg} (code {e}) {detail}".rstrip())

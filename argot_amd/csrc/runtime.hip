@@ -26,14 +26,22 @@ void set_err(int e) {
   if (t_err == ARGOT_OK) t_err = e;
 }
 
-#define HIP_OK(expr)                                        \
-  do {                                                      \
-    hipError_t _e = (expr);                                 \
-    if (_e != hipSuccess) {                                 \
-      set_err(_e == hipErrorOutOfMemory ? ARGOT_ERR_NOMEM   \
-                                        : ARGOT_ERR_HIP);   \
-      return;                                               \
-    }                                                       \
+thread_local char t_detail[256] = "";
+
+void set_hip_err(hipError_t e, int line) {
+  if (t_err == ARGOT_OK)
+    snprintf(t_detail, sizeof t_detail, "runtime.hip:%d: %s (%d)", line, hipGetErrorString(e),
+             (int)e);
+  set_err(e == hipErrorOutOfMemory ? ARGOT_ERR_NOMEM : ARGOT_ERR_HIP);
+}
+
+#define HIP_OK(expr)                      \
+  do {                                    \
+    hipError_t _e = (expr);               \
+    if (_e != hipSuccess) {               \
+      set_hip_err(_e, __LINE__);          \
+      return;                             \
+    }                                     \
   } while (0)
 
 // Options (library controls).
@@ -236,6 +244,8 @@ double argot_hip_internal_helium_fact() { return g_helium_fact; }
 extern "C" {
 
 int argot_hip_last_error(void) { return t_err; }
+
+const char* argot_hip_last_error_detail(void) { return t_err == ARGOT_OK ? "" : t_detail; }
 
 const char* argot_hip_error_string(int code) {
   switch (code) {
@@ -587,16 +597,16 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   a.rec_cap = capture ? g_capture_cap : 0;
   const int nblock = std::max(1, g.ncu * g_blocks_per_cu);
   // Device work of the step: per-cell FP32 densities, rates zeroed
-  // (zero_out_photoion_rate_cuda), counters reset, the fused persistent
-  // kernel.  All kernels (no memset nodes), so the sequence can live in a
-  // HIP graph; the graph is re-captured when any launch argument changes.
+  // (zero_out_photoion_rate_cuda), counters reset (small kernels), then the
+  // fused persistent kernel, timed alone between ev_mid and ev1.  The fused
+  // launch is replayed from a HIP graph, captured again only when a launch
+  // argument changes (the stream events stay outside the graph).
+  HIP_OK(hipEventRecord(g.ev0, g.stream));
+  HIP_OK(launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream));
+  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
+  HIP_OK(launch_reset_counters(g.ctr.p, g.stream));
   auto enqueue = [&]() -> hipError_t {
-    hipError_t e = launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream);
-    if (e == hipSuccess) e = launch_zero_rates(g.mesh, g.ncell, g.stream);
-    if (e == hipSuccess) e = launch_reset_counters(g.ctr.p, g.stream);
-    if (e == hipSuccess) e = hipEventRecord(g.ev_mid, g.stream);   // fused kernel alone
-    if (e == hipSuccess) e = launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream);
-    return e;
+    return launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream);
   };
   const bool use_graph = g_use_graph && !capture;
   if (use_graph) {
@@ -624,11 +634,11 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
       g.graph_key = key;
       g.stats_graph_captures++;
     }
-    HIP_OK(hipEventRecord(g.ev0, g.stream));
+    HIP_OK(hipEventRecord(g.ev_mid, g.stream));
     HIP_OK(hipGraphLaunch(g.graph_exec, g.stream));
     HIP_OK(hipEventRecord(g.ev1, g.stream));
   } else {
-    HIP_OK(hipEventRecord(g.ev0, g.stream));
+    HIP_OK(hipEventRecord(g.ev_mid, g.stream));
     HIP_OK(enqueue());
     HIP_OK(hipEventRecord(g.ev1, g.stream));
   }

@@ -4,12 +4,17 @@ Workload (the size the metric is quoted on): 256^3 lognormal H+He mesh,
 4096 sources in 8 Gaussian clumps (sigma = 8 cells), 1e5 K blackbody spectra,
 256 frequency bins, theta = 0.7, one rank owning the whole box.  A step is one
 step_radiation_tree call through the C-ABI: tree build on the host, then the
-fused GPU kernel (tree walk, segment, optical-depth walk, photon-conserving
-rates, per-cell sums).  Inputs are resident in HBM before timing starts.
+device work (field packing, rate zeroing, the fused persistent kernel: tree
+walk, segment, optical-depth walk, photon-conserving rates, per-cell sums)
+replayed as one HIP graph.  Inputs are resident in HBM before timing starts.
+The autotuner picks the tile shape and grid size during the untimed warmup.
 
 Prints one JSON line: segments/s (one segment per ray on one rank), the
-fused kernel's roofline against HBM, and the CPU oracle timed on a bounded
-sample of the same workload on this host's cores.
+fused kernel's roofline against HBM, the CPU oracle timed on a bounded
+sample of the same workload on this host's cores, and two side measurements:
+the MLP rate surrogate (training, validated accuracy, rays it may serve at
+1e-3, step speed with it) and the ART diffuse transfer on 128^3 x 768
+directions.
 """
 import argparse
 import json
@@ -66,6 +71,65 @@ def cpu_baseline(w, seconds_target=15.0):
     }
 
 
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def surrogate_leg(step, plain_seg_s, nray):
+    """Train the rate network on the C4 grid and on the step's own grid
+    (bf16-MFMA training on the GPU), report validated accuracy, then run the
+    step with rate mode 1 (the network where validated within 1e-3, the
+    exact integral elsewhere)."""
+    from argot_amd import surrogate as S
+    out = {}
+    nu, cs, spec = S.spectral_grid(128, S.C4_LOG_NU_MAX)
+    rep = S.train(S.Config(nu, cs, spec, nsample=1 << 22, batch=16384, nstep=20000,
+                           trust_n=32, trust_tol=1e-3))
+    out["c4_128bin"] = {k: rep[k] for k in ("ms_train", "err_median", "err_p99", "err_max",
+                                            "trusted_frac", "loss_last")}
+    log(f"surrogate C4: {out['c4_128bin']}")
+    nu, cs, spec = S.step_grid(step.freq, step.cs,
+                               np.ctypeslib.as_array(step.src[0].photon_rate))
+    rep = S.train(S.Config(nu, cs, spec, nsample=1 << 22, batch=16384, nstep=20000,
+                           trust_n=32, trust_tol=1e-3))
+    out["step_grid"] = {k: rep[k] for k in ("ms_train", "err_median", "err_p99", "err_max",
+                                            "trusted_frac", "loss_last")}
+    step.rate_mode = 1
+    step.step()
+    t0 = time.perf_counter()
+    st = step.step()
+    ms = 1000.0 * (time.perf_counter() - t0)
+    step.rate_mode = 0
+    out["step_rate_mode1"] = {"segments_per_s": nray / (ms * 1e-3),
+                              "ms_per_step": ms, "rays_served": int(st["nsurrogate"]),
+                              "vs_exact": (nray / (ms * 1e-3)) / plain_seg_s,
+                              "tolerance": 1e-3}
+    log(f"surrogate step: {out['step_rate_mode1']}")
+    return out
+
+
+def art_leg():
+    """ART diffuse transfer, 128^3 lognormal mesh, 768 HEALPix directions."""
+    from argot_amd import art as A
+    from argot_amd import ic
+    w = ic.art128()
+    st = A.DiffuseStep(w, nside=8, batch=32)
+    st.step()
+    t0 = time.perf_counter()
+    s = st.step()
+    ms = 1000.0 * (time.perf_counter() - t0)
+    st.close()
+    # per crossing: absorption + source (3 bands) 24 B read, band intensities
+    # 12 B and path length 4 B read-modify-write (32 B)
+    bytes_ = s["ncross"] * (24 + 32)
+    return {"workload": "art128: 128^3, 768 directions, 3 bands", "ncross": int(s["ncross"]),
+            "ms_per_pass": ms, "ms_trace": s["ms_trace"], "ms_rates": s["ms_gh"],
+            "crossings_per_s": s["ncross"] / (s["ms_trace"] * 1e-3),
+            "trace_roofline": {"bound": "hbm", "achieved": bytes_ / (s["ms_trace"] * 1e-3) / 1e9,
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": bytes_ / (s["ms_trace"] * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,7 +137,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="clumped256")
     ap.add_argument("--kahan", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tile", default=None, help="fixed target tile, e.g. 4x4x4 (skips tuning)")
+    ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the surrogate and ART legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -82,18 +150,34 @@ def main():
 
     from argot_amd import ic
     from argot_amd import radiation as R
+    from argot_amd._lib import check, lib
 
     t0 = time.perf_counter()
     w = ic.WORKLOADS[args.workload]()
     step = R.PointSourceStep(w, tau_mode=1 if args.kahan else 0)
-    setup_s = time.perf_counter() - t0
-    print(f"# setup {setup_s:.1f} s: {args.workload} n={w.n} nsrc={w.nsrc}", file=sys.stderr,
-          flush=True)
+    log(f"setup {time.perf_counter() - t0:.1f} s: {args.workload} n={w.n} nsrc={w.nsrc}")
 
     for i in range(args.warmup):
         st = step.step()
-        print(f"# warmup {i}: {st['nray']} segments, kernel {st['ms_kernel']:.1f} ms, "
-              f"total {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+        log(f"warmup {i}: {st['nray']} segments, kernel {st['ms_kernel']:.1f} ms, "
+            f"total {st['ms_total']:.1f} ms")
+    tuned = None
+    if args.tile or args.blocks_per_cu:
+        if args.tile:
+            lib().argot_hip_set_tile(*[int(v) for v in args.tile.split("x")])
+            check("argot_hip_set_tile")
+        if args.blocks_per_cu:
+            lib().argot_hip_set_blocks_per_cu(args.blocks_per_cu)
+        tuned = {"tile": args.tile, "blocks_per_cu": args.blocks_per_cu, "candidates": 0}
+        step.step()
+    elif not args.no_tune:
+        rows, best = step.autotune()
+        tuned = {"tile": list(rows[best]["tile"]), "blocks_per_cu": rows[best]["blocks_per_cu"],
+                 "candidates": len(rows),
+                 "ms_kernel_range": [min(r["ms_kernel"] for r in rows),
+                                     max(r["ms_kernel"] for r in rows)]}
+        log(f"autotune: {tuned}")
+        step.step()   # graph capture for the selected configuration
     ker = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -102,8 +186,8 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = 1000.0 * wall / args.steps
     nray, ncross, ncell = st["nray"], st["ncross"], st["ntarget"]
-    print(f"# timed: {args.steps} steps, {ms_step:.1f} ms/step, kernel {np.mean(ker):.1f} ms, "
-          f"{ncross / max(nray, 1):.1f} crossings/segment", file=sys.stderr, flush=True)
+    log(f"timed: {args.steps} steps, {ms_step:.1f} ms/step, kernel {np.mean(ker):.1f} ms, "
+        f"{ncross / max(nray, 1):.1f} crossings/segment")
 
     kernel_ms = float(np.mean(ker))
     bytes_launch = algorithmic_bytes(nray, ncross, ncell)
@@ -117,14 +201,13 @@ def main():
             rec = json.load(f).get("fused_step_kernel", {})
         if rec.get("workload") == args.workload:
             traffic = rec.get("traffic_bytes")
-    from argot_amd._lib import check, lib
     copy_gbs = lib().argot_hip_copy_probe(1 << 30, 10)
     check("argot_hip_copy_probe")
-    print(f"# copy probe: {copy_gbs:.0f} GB/s (1 GiB copy, read + write)", file=sys.stderr,
-          flush=True)
+    log(f"copy probe: {copy_gbs:.0f} GB/s (1 GiB copy, read + write)")
+    seg_s = nray / (ms_step * 1e-3)
     out = {
         "metric": "ray segments/s (point-source step, exact rates)",
-        "value": nray / (ms_step * 1e-3),
+        "value": seg_s,
         "unit": "segments/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -138,13 +221,18 @@ def main():
         "config": {"workload": f"{args.workload}: {w.n}^3 mesh, {w.nsrc} sources, "
                                "256 nu bins, theta 0.7, one rank",
                    "segments_per_step": int(nray), "crossings_per_step": int(ncross),
-                   "tau_sum": "kahan" if args.kahan else "plain"},
+                   "tau_sum": "kahan" if args.kahan else "plain", "hip_graph": True,
+                   "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "fused_step_kernel", "kernel_ms": kernel_ms,
                      "bytes_per_launch": int(bytes_launch),
                      "measured_copy_gbs": copy_gbs},
     }
+    if not args.no_extras:
+        out["surrogate"] = surrogate_leg(step, seg_s, nray)
+        out["art"] = art_leg()
+        log(f"art: {out['art']}")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

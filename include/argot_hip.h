@@ -345,6 +345,9 @@ enum argot_hip_status {
 
 /* Status of the last entry point called on this thread (0 = OK). */
 int argot_hip_last_error(void);
+/* For ARGOT_ERR_HIP / ARGOT_ERR_NOMEM from the step and transfer entry
+ * points: the failing call's source line and HIP error string. */
+const char *argot_hip_last_error_detail(void);
 const char *argot_hip_error_string(int code);
 
 /* n_He/n_H used for the helium number densities (reference: the
@@ -453,9 +456,9 @@ void argot_hip_set_blocks_per_cu(int n);
  * each a power of two (default 4x4x4). */
 void argot_hip_set_tile(int tx, int ty, int tz);
 
-/* 1 (default): step_radiation_tree replays its device work (field packing,
- * rate zeroing, counter reset, fused kernel) as one HIP graph, captured
- * again only when a launch argument changes; 0: plain stream launches. */
+/* 1 (default): step_radiation_tree replays its persistent fused launch
+ * from a HIP graph, captured again only when a launch argument changes;
+ * 0: plain stream launch. */
 void argot_hip_set_graph(int on);
 /* Graph captures so far (tests). */
 int64_t argot_hip_graph_captures(void);
