@@ -34,6 +34,7 @@ EXPORTS = [
     "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
     "argot_hip_set_diffuse_exp64", "argot_hip_set_tile", "argot_hip_set_graph",
     "argot_hip_graph_captures", "argot_hip_autotune_step", "argot_hip_last_error_detail",
+    "smooth_photoion_rate",
 ]
 
 _lib = None
@@ -118,6 +119,7 @@ def lib():
               "argot_hip_set_diffuse_exp64"):
         getattr(L, n).argtypes = [C.c_int]
     L.argot_hip_diffuse_stats.argtypes = [P(abi.DiffuseStats)]
+    L.smooth_photoion_rate.argtypes = [C.c_void_p, rp, cm, cp, C.c_void_p]
     L.argot_hip_set_tile.argtypes = [C.c_int, C.c_int, C.c_int]
     L.argot_hip_set_graph.argtypes = [C.c_int]
     L.argot_hip_graph_captures.restype = C.c_int64

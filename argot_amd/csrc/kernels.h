@@ -31,6 +31,10 @@ hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* 
 hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
                                   void* mesh, hipStream_t s);
 hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s);
+// smooth_photoion_rate on one rank; tmp_in/tmp_out: 6*ncell floats each.
+// On return (stream order) tmp_out holds the smoothed rates as six planes.
+hipError_t launch_smooth_rates(void* mesh, int nx, int ny, int nz, float* tmp_in, float* tmp_out,
+                               hipStream_t s);
 hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblock,
                              hipStream_t s);
 hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,

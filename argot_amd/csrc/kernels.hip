@@ -84,6 +84,59 @@ __global__ void walk_segments_kernel(GridDev g, const float4* __restrict__ nf,
 // species s >= NS have sigma_s = 0 in these bins, so their optical depth
 // enters only through the constant cst).  Tables are read with scalar loads
 // (wave-uniform bin index); sums run in FP32 over 32 bins, then in FP64.
+// sinh(y)/y = sum_k y^2k/(2k+1)! through y^14: truncation < 1e-10 relative
+// for y <= 2.
+__device__ __forceinline__ float sinhc_small(float y) {
+  const float y2 = y * y;
+  float p = __builtin_fmaf(y2, 1.0f / 1307674368000.0f, 1.0f / 6227020800.0f);
+  p = __builtin_fmaf(y2, p, 1.0f / 39916800.0f);
+  p = __builtin_fmaf(y2, p, 1.0f / 362880.0f);
+  p = __builtin_fmaf(y2, p, 1.0f / 5040.0f);
+  p = __builtin_fmaf(y2, p, 1.0f / 120.0f);
+  p = __builtin_fmaf(y2, p, 1.0f / 6.0f);
+  return __builtin_fmaf(y2, p, 1.0f);
+}
+
+// Bins [k0, k1) when every in-cell optical depth of the wavefront's rays is
+// at most 4: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y
+// with y = dtau/2, one exponential per bin.
+template <int NS, typename SpecF>
+__device__ __forceinline__ void rate_bins_thin(const RateTables& rt, int k0, int k1, float cst,
+                                               const float n[3], const float d[3], SpecF spec,
+                                               double G[3], double H[3]) {
+  const int ng = rt.ngrid;
+  for (int c = k0; c < k1; c += 32) {
+    float g[NS], h[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) g[s] = h[s] = 0.0f;
+    const int ce = min(c + 32, k1);
+#pragma unroll 4
+    for (int k = c; k < ce; k++) {
+      float sg[NS], hs[NS];
+      float tau = cst, dt = 0.0f;
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        sg[s] = ldc(rt.sig + s * ng + k);
+        hs[s] = ldc(rt.hsig + s * ng + k);
+        tau = __builtin_fmaf(n[s], ldc(rt.dsig + s * ng + k), tau);
+        dt = __builtin_fmaf(d[s], sg[s], dt);
+      }
+      const float y = 0.5f * dt;
+      const float b = spec(k) * __expf(-(tau + y)) * sinhc_small(y);
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        g[s] = __builtin_fmaf(sg[s], b, g[s]);
+        h[s] = __builtin_fmaf(hs[s], b, h[s]);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      G[s] += g[s];
+      H[s] += h[s];
+    }
+  }
+}
+
 template <int NS, typename SpecF>
 __device__ __forceinline__ void rate_bins(const RateTables& rt, int k0, int k1, float cst,
                                           const float n[3], const float d[3], SpecF spec,
@@ -135,9 +188,17 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   const float c1 = (float)(-(rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2]));
   const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
   double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
-  rate_bins<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
-  rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
-  rate_bins<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
+  // largest in-cell optical depth of this ray over the bins
+  const double dmax = rr.d[0] * rt.cs_max[0] + rr.d[1] * rt.cs_max[1] + rr.d[2] * rt.cs_max[2];
+  if (__all(dmax <= 4.0)) {
+    rate_bins_thin<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
+    rate_bins_thin<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
+    rate_bins_thin<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
+  } else {
+    rate_bins<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
+    rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
+    rate_bins<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
+  }
   const double pre = scale * exp(-tsh);
   const double p0 = rr.r[0] * rr.d[0] * pre;
   const double p1 = rr.r[1] * rr.d[1] * pre;
@@ -449,6 +510,74 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     atomicAdd(&a.ctr->nfallback, sf);
     if (SURR) atomicAdd(&a.ctr->nsurrogate, ss);
   }
+}
+
+// smooth_photoion_rate (smooth_photoion_rate.c:1058-1090) on one rank.
+// The six prev_chem rates go to SoA planes, each plane is smoothed by the
+// 27-point stencil (central weight 30, 1/56; ghost cells are the nearest
+// interior cell), and the result goes back into prev_chem.
+__global__ void gather_rates_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
+                                    float* __restrict__ soa) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float* r = mesh_f(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) soa[k * ncell + c] = r[k];
+}
+
+__global__ void smooth_rates_kernel(int nx, int ny, int nz, const float* __restrict__ in,
+                                    float* __restrict__ out) {
+  const int64_t ncell = (int64_t)nx * ny * nz;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 6 * ncell) return;
+  const int k = (int)(i / ncell);
+  const int64_t c = i - (int64_t)k * ncell;
+  const int iz = (int)(c % nz), iy = (int)((c / nz) % ny), ix = (int)(c / ((int64_t)nz * ny));
+  const float* p = in + (int64_t)k * ncell;
+  // C evaluation order of the reference's sum: thirteen float additions,
+  // then the double central term, then double additions
+  float s = 0.0f;
+  double d = 0.0;
+  int term = 0;
+#pragma unroll
+  for (int dx = -1; dx <= 1; dx++) {
+    const int jx = min(max(ix + dx, 0), nx - 1);
+#pragma unroll
+    for (int dy = -1; dy <= 1; dy++) {
+      const int jy = min(max(iy + dy, 0), ny - 1);
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const int jz = min(max(iz + dz, 0), nz - 1);
+        const float v = p[(int64_t)jz + (int64_t)nz * ((int64_t)jy + (int64_t)ny * jx)];
+        if (term == 0) s = v;
+        else if (term < 13) s = s + v;
+        else if (term == 13) d = (double)s + 30.0 * (double)v;
+        else d = d + (double)v;
+        term++;
+      }
+    }
+  }
+  out[i] = (float)((1.0 / (30.0 + 26.0)) * d);
+}
+
+__global__ void scatter_rates_kernel(unsigned char* __restrict__ mesh, int64_t ncell,
+                                     const float* __restrict__ soa) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  float* r = mesh_fw(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) r[k] = soa[k * ncell + c];
+}
+
+hipError_t launch_smooth_rates(void* mesh, int nx, int ny, int nz, float* tmp_in, float* tmp_out,
+                               hipStream_t s) {
+  const int64_t ncell = (int64_t)nx * ny * nz;
+  const unsigned nb = (unsigned)((ncell + 255) / 256);
+  gather_rates_kernel<<<nb, 256, 0, s>>>(static_cast<unsigned char*>(mesh), ncell, tmp_in);
+  smooth_rates_kernel<<<(unsigned)((6 * ncell + 255) / 256), 256, 0, s>>>(nx, ny, nz, tmp_in,
+                                                                          tmp_out);
+  scatter_rates_kernel<<<nb, 256, 0, s>>>(static_cast<unsigned char*>(mesh), ncell, tmp_out);
+  return hipGetLastError();
 }
 
 // Plain device-to-device copy (16 B per thread per iteration, grid-stride):

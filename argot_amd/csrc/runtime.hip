@@ -108,6 +108,7 @@ struct Ctx {
   DevBuf<double> per_ray;
   DevBuf<int64_t> target;
   DevBuf<float> rate_tab;
+  DevBuf<float> smooth_in, smooth_out;   // smooth_photoion_rate planes
   HostTree tree;
   argot_step_stats stats{};
   // step graph
@@ -667,6 +668,36 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
 // streaming kernel; returns the best GB/s (read + write bytes / time).
+// smooth_photoion_rate, GPU variant (src/prototype.h:89,
+// src/smooth_photoion_rate.c:27-1588) on one rank: the rates in
+// mesh_dev[].prev_chem are smoothed on the device and, as the reference
+// does, also written into the host mesh's prev_chem (when mesh != NULL).
+void smooth_photoion_rate(struct fluid_mesh* mesh, struct run_param* this_run,
+                          struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                          struct mpi_param* this_mpi) {
+  (void)this_cuda;
+  (void)this_mpi;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  if (this_run->nnode_x > 1 || this_run->nnode_y > 1 || this_run->nnode_z > 1)
+    return set_err(ARGOT_ERR_ARG);   // the halo exchange of several ranks is not built
+  HIP_OK(g.smooth_in.reserve(6 * (size_t)g.ncell));
+  HIP_OK(g.smooth_out.reserve(6 * (size_t)g.ncell));
+  HIP_OK(launch_smooth_rates(g.mesh, g.nx, g.ny, g.nz, g.smooth_in.p, g.smooth_out.p, g.stream));
+  if (mesh) {
+    std::vector<float> soa(6 * (size_t)g.ncell);
+    HIP_OK(hipMemcpyAsync(soa.data(), g.smooth_out.p, sizeof(float) * soa.size(),
+                          hipMemcpyDeviceToHost, g.stream));
+    HIP_OK(hipStreamSynchronize(g.stream));
+    for (int64_t c = 0; c < g.ncell; c++) {
+      float* r = &mesh[c].prev_chem.GammaHI;
+      for (int k = 0; k < 6; k++) r[k] = soa[(size_t)k * g.ncell + c];
+    }
+  } else {
+    HIP_OK(hipStreamSynchronize(g.stream));
+  }
+}
+
 // Autotuner: runs the step on the caller's inputs for each target-tile shape
 // (the 64 targets of a wavefront share every tree decision, so the shape
 // sets how coherent the wavefront's rays are) and persistent-grid size,

@@ -125,6 +125,14 @@ class PointSourceStep:
         check("step_radiation_tree")
         return self.stats()
 
+    def smooth(self):
+        """smooth_photoion_rate on the device rates; the host mesh receives
+        the smoothed rates too."""
+        L = lib()
+        L.smooth_photoion_rate(self.mesh, C.byref(self.run), C.byref(self.cmem),
+                               C.byref(self.cpar), None)
+        check("smooth_photoion_rate")
+
     def autotune(self):
         """Time every tile shape / grid size on this step's inputs, keep the
         fastest (argot_hip_autotune_step); returns the candidates and the

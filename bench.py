@@ -142,11 +142,16 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the surrogate and ART legs")
+    ap.add_argument("--art-only", action="store_true",
+                    help="run only the ART leg and print its JSON (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.art_only:
+        print(json.dumps(art_leg()), flush=True)
+        return
 
     from argot_amd import ic
     from argot_amd import radiation as R

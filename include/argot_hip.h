@@ -232,6 +232,14 @@ void calc_photoion_rate_cuda(struct cuda_mem_space *cuda_mem,
 void zero_out_photoion_rate_cuda(struct cuda_mem_space *cuda_mem,
                                  struct cuda_param *this_cuda,
                                  struct run_param *this_run);
+/* replaces smooth_photoion_rate (GPU variant), src/prototype.h:89 and
+ * src/smooth_photoion_rate.c:27-1588, on one rank: the 27-point smoothing
+ * (central weight 30, ghost cells = nearest interior cell) of the rates in
+ * mesh_dev[].prev_chem, on the device; the smoothed rates are also written
+ * into the host mesh's prev_chem when mesh != NULL, as the reference does. */
+void smooth_photoion_rate(struct fluid_mesh *mesh, struct run_param *this_run,
+                          struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                          struct mpi_param *this_mpi);
 /* replaces merge_photoion_data, src/merge_photoion_data.cu:16-83:
  * mesh_dev[].prev_chem += host mesh[].prev_chem rates. */
 void merge_photoion_data(struct fluid_mesh *mesh,

@@ -180,6 +180,11 @@ uint64_t orc_ray_records(const orc_grid *g, const orc_fields *f,
 void orc_thin_kernel(int ngrid, const double *nu, const float *cs,
                      const double *spec, const double *tau_th, double *out4);
 
+/* smooth_photoion_rate on one rank (src/smooth_photoion_rate.c:1058-1090):
+ * 27-point stencil, central weight 30, ghost cells = nearest interior
+ * cell; six rates per cell in mesh order. */
+void orc_smooth_rates(int nx, int ny, int nz, const float *in, float *out);
+
 /* ------------------------------------------------------------------ */
 /* ART diffuse-photon transfer (art.c), one rank                        */
 /* ------------------------------------------------------------------ */

@@ -366,3 +366,13 @@ def art_run(g, nside, dens, chem, uenetok, heatcool=True):
                            np.ascontiguousarray(dens, np.float32).reshape(-1),
                            np.ascontiguousarray(chem, np.float32).reshape(-1), uenetok, rates)
     return rates.reshape(ncell, 6), nc
+
+
+def smooth_rates(rates, shape):
+    """smooth_photoion_rate on one rank: rates [ncell, 6] in mesh order."""
+    L = lib()
+    L.orc_smooth_rates.argtypes = [C.c_int, C.c_int, C.c_int, f32p, f32p]
+    r = np.ascontiguousarray(rates, np.float32).reshape(-1)
+    out = np.zeros_like(r)
+    L.orc_smooth_rates(shape[0], shape[1], shape[2], r, out)
+    return out.reshape(-1, 6)

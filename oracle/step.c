@@ -158,3 +158,39 @@ uint64_t orc_ray_records(const orc_grid *g, const orc_fields *f,
     free(b.kind); free(b.id); free(b.pos);
     return w;
 }
+
+/* smooth_photoion_rate, src/smooth_photoion_rate.c:1058-1090 (the 27-point
+ * stencil of every rate, central weight 30, normalised by 1/56) on one
+ * rank: ghost cells copy the nearest interior cell (the source_rank < 0
+ * branches, e.g. :180-184, :222-226, :1052-1055).  in/out: six rates per
+ * cell in mesh order.  The first thirteen terms add in float; the central
+ * term is a double product, so the rest of the sum is double. */
+void orc_smooth_rates(int nx, int ny, int nz, const float *in, float *out)
+{
+    const double normal = 1.0 / (30.0 + 26.0);
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int iz = 0; iz < nz; iz++) {
+                const long c = iz + (long)nz * (iy + (long)ny * ix);
+                for (int r = 0; r < 6; r++) {
+                    float s = 0.0f;
+                    double d = 0.0;
+                    int term = 0;
+                    for (int dx = -1; dx <= 1; dx++)
+                        for (int dy = -1; dy <= 1; dy++)
+                            for (int dz = -1; dz <= 1; dz++) {
+                                int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                                jx = jx < 0 ? 0 : (jx >= nx ? nx - 1 : jx);
+                                jy = jy < 0 ? 0 : (jy >= ny ? ny - 1 : jy);
+                                jz = jz < 0 ? 0 : (jz >= nz ? nz - 1 : jz);
+                                const float v = in[6 * (jz + (long)nz * (jy + (long)ny * jx)) + r];
+                                if (term == 0) s = v;
+                                else if (term < 13) s = s + v;
+                                else if (term == 13) d = (double)s + 30.0 * (double)v;
+                                else d = d + (double)v;
+                                term++;
+                            }
+                    out[6 * c + r] = (float)(normal * d);
+                }
+            }
+}
