@@ -128,14 +128,24 @@ __device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3
 struct Dda {
   int ix, iy, iz;        // current cell
   int ixe, iye, ize;     // end cell (clamped)
-  uint32_t idx;          // linear index of the current cell
+  uint32_t idxe;         // linear index of the end cell
   float xo, yo, zo;      // direction cosines (with the TINY guards)
   double rxo, ryo, rzo;  // 1/xo etc. in double (exact-division helper)
   float xsf, ysf, zsf;   // x_step etc. as float (0 or 1)
   int xn, yn, zn;        // +-1
-  uint32_t sx, sy, sz;   // linear-index step for each axis (xn*ny*nz, yn*nz, zn)
   float xc, yc, zc;      // position in local coordinates
 };
+
+// Linear index iz + nz (iy + ny ix) of a cell inside the mesh, from two
+// 24-bit multiply-adds (mesh sizes and indices are below 2^24).  The inner
+// one is written out: left to itself the compiler folds it into a
+// quarter-rate 64-bit multiply-add.  ny and nz are kernel arguments, so
+// they sit in scalar registers.
+__device__ __forceinline__ uint32_t cell_index(const GridDev& g, int ix, int iy, int iz) {
+  uint32_t row;
+  asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(row) : "v"(ix), "s"(g.ny), "v"(iy));
+  return __umul24(row, (uint32_t)g.nz) + (uint32_t)iz;
+}
 
 __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   int ixe = (int)((e.x - g.xmin) / g.dx);
@@ -150,7 +160,7 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->ix = max(min(ix, g.nx - 1), 0);
   d->iy = max(min(iy, g.ny - 1), 0);
   d->iz = max(min(iz, g.nz - 1), 0);
-  d->idx = (uint32_t)d->iz + (uint32_t)g.nz * ((uint32_t)d->iy + (uint32_t)g.ny * d->ix);
+  d->idxe = cell_index(g, d->ixe, d->iye, d->ize);
 
   float dx = e.x - s.x, dy = e.y - s.y, dz = e.z - s.z;
   float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kWalkTiny);
@@ -178,19 +188,19 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->xn = px ? 1 : -1;
   d->yn = py ? 1 : -1;
   d->zn = pz ? 1 : -1;
-  d->sx = (uint32_t)(d->xn * (int)g.nyz);
-  d->sy = (uint32_t)(d->yn * g.nz);
-  d->sz = (uint32_t)d->zn;
   d->xc = s.x - g.xmin;
   d->yc = s.y - g.ymin;
   d->zc = s.z - g.zmin;
 }
 
-__device__ __forceinline__ bool dda_active(const GridDev& g, const Dda& d) {
-  const bool moving = (d.ix != d.ixe) | (d.iy != d.iye) | (d.iz != d.ize);
+// The walk continues while the current cell lies inside the mesh and is not
+// the end cell; *idx receives the current cell's linear index (meaningful
+// only when the cell is inside).
+__device__ __forceinline__ bool dda_active(const GridDev& g, const Dda& d, uint32_t* idx) {
   const bool inside = ((unsigned)d.ix < (unsigned)g.nx) & ((unsigned)d.iy < (unsigned)g.ny) &
                       ((unsigned)d.iz < (unsigned)g.nz);
-  return moving & inside;
+  *idx = cell_index(g, d.ix, d.iy, d.iz);
+  return inside & (*idx != d.idxe);
 }
 
 // One step of calc_optical_depth.c:163-179; returns the chord length.
@@ -206,7 +216,6 @@ __device__ __forceinline__ float dda_step(const GridDev& g, Dda* d) {
   d->ix += ax ? d->xn : 0;
   d->iy += ay ? d->yn : 0;
   d->iz += az ? d->zn : 0;
-  d->idx += (ax ? d->sx : 0u) + (ay ? d->sy : 0u) + (az ? d->sz : 0u);
   d->xc = d->xc + rmin * d->xo;
   d->yc = d->yc + rmin * d->yo;
   d->zc = d->zc + rmin * d->zo;
@@ -248,14 +257,15 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
-  bool act = dda_active(g, d);
-  float4 n = nf[act ? d.idx : 0u];
+  uint32_t idx;
+  bool act = dda_active(g, d, &idx);
+  float4 n = nf[act ? idx : 0u];
   while (act) {
-    if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+    if (HASH) h = (h ^ (uint64_t)idx) * 0x100000001b3ull;
     nc++;
     const float rmin = dda_step(g, &d);
-    const bool nact = dda_active(g, d);
-    const float4 nn = nf[nact ? d.idx : 0u];
+    const bool nact = dda_active(g, d, &idx);
+    const float4 nn = nf[nact ? idx : 0u];
     if (KAHAN) {
       kahan_add(t0, c0, rmin * n.x);
       kahan_add(t1, c1, rmin * n.y);
@@ -269,7 +279,7 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
     act = nact;
   }
   const float depth = dda_depth(g, d, e);
-  const uint32_t ie = (uint32_t)d.ize + (uint32_t)g.nz * ((uint32_t)d.iye + (uint32_t)g.ny * d.ixe);
+  const uint32_t ie = d.idxe;
   n = nf[ie];
   if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
   nc++;
