@@ -34,7 +34,9 @@ EXPORTS = [
     "argot_hip_set_diffuse_batch", "argot_hip_set_diffuse_kahan", "argot_hip_diffuse_stats",
     "argot_hip_set_diffuse_exp64", "argot_hip_set_tile", "argot_hip_set_graph",
     "argot_hip_graph_captures", "argot_hip_autotune_step", "argot_hip_last_error_detail",
-    "smooth_photoion_rate",
+    "smooth_photoion_rate", "step_chemistry_gpu", "step_chemistry", "calc_dtime_gpu",
+    "calc_timestep_chem", "update_chemistry_gpu", "argot_hip_chem_stats",
+    "argot_hip_set_iterations",
 ]
 
 _lib = None
@@ -120,6 +122,16 @@ def lib():
         getattr(L, n).argtypes = [C.c_int]
     L.argot_hip_diffuse_stats.argtypes = [P(abi.DiffuseStats)]
     L.smooth_photoion_rate.argtypes = [C.c_void_p, rp, cm, cp, C.c_void_p]
+    fp = P(C.c_float)
+    L.step_chemistry_gpu.argtypes = [cm, cp, rp, fp, fp, C.c_float]
+    L.step_chemistry.argtypes = [cm, cp, rp, C.c_float, fp, fp]
+    L.calc_dtime_gpu.argtypes = [cm, cp, rp]
+    L.calc_dtime_gpu.restype = C.c_double
+    L.calc_timestep_chem.argtypes = [cm, cp, rp]
+    L.calc_timestep_chem.restype = C.c_double
+    L.update_chemistry_gpu.argtypes = [cm, cp, rp]
+    L.argot_hip_chem_stats.argtypes = [P(abi.ChemStats)]
+    L.argot_hip_set_iterations.argtypes = [C.c_int]
     L.argot_hip_set_tile.argtypes = [C.c_int, C.c_int, C.c_int]
     L.argot_hip_set_graph.argtypes = [C.c_int]
     L.argot_hip_graph_captures.restype = C.c_int64

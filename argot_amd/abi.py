@@ -119,7 +119,13 @@ class CudaMemSpace(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [("ntarget", C.c_uint64), ("nray", C.c_uint64), ("ncross", C.c_uint64),
                 ("nfallback", C.c_uint64), ("ms_build", C.c_double), ("ms_kernel", C.c_double),
-                ("ms_total", C.c_double), ("nsurrogate", C.c_uint64)]
+                ("ms_total", C.c_double), ("nsurrogate", C.c_uint64),
+                ("iterations", C.c_uint64), ("ms_chem", C.c_double)]
+
+
+class ChemStats(C.Structure):
+    _fields_ = [("ncell", C.c_uint64), ("nsub", C.c_uint64), ("niter", C.c_uint64),
+                ("ms_kernel", C.c_double)]
 
 
 class SurrogateConfig(C.Structure):

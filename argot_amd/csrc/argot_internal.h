@@ -91,4 +91,23 @@ struct StepCounters {
   int queue[kNumQueues];
 };
 
+// Run-time parameters of the chemistry (from struct run_param, plus the
+// helium number fraction).
+struct ChemParams {
+  double lunit, munit, tunit, uenetok, denstonh, helium_fact;
+  float anow, znow;
+};
+
+// Results of one chemistry launch: maxima over cells as float bits (the
+// values are non-negative, so unsigned order is float order), the minimum
+// safe step, and work counts.
+struct ChemCounters {
+  unsigned int diff_chem;
+  unsigned int diff_uene;
+  unsigned int dt_min;
+  unsigned int pad;
+  unsigned long long nsub;
+  unsigned long long niter;
+};
+
 }  // namespace argot

@@ -40,4 +40,11 @@ hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblo
 hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,
                              hipStream_t s);
 
+// Chemistry (chemistry.hip).  Each launch first resets *ctr on the stream.
+hipError_t launch_chem_step(void* mesh, int64_t ncell, const ChemParams& p, float dtime,
+                            ChemCounters* ctr, hipStream_t s);
+hipError_t launch_chem_dtime(const void* mesh, int64_t ncell, const ChemParams& p,
+                             ChemCounters* ctr, hipStream_t s);
+hipError_t launch_chem_update(void* mesh, int64_t ncell, hipStream_t s);
+
 }  // namespace argot

@@ -2,6 +2,7 @@
 // (src/prototype.h:86-130) on one MI355X, the fused step_radiation_tree,
 // and the library controls declared in include/argot_hip.h.
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -52,6 +53,7 @@ float g_theta = 0.7f;                  // ARGOT_THETA_CRIT, run_param.h:80
 int g_blocks_per_cu = 16;
 int g_tile_bits[3] = {2, 2, 2};        // 4x4x4 targets per wavefront
 int g_use_graph = 1;
+int g_iterations = 0;                  // 0: one point-source pass
 
 // Everything a captured step graph depends on.
 struct GraphKey {
@@ -109,6 +111,9 @@ struct Ctx {
   DevBuf<int64_t> target;
   DevBuf<float> rate_tab;
   DevBuf<float> smooth_in, smooth_out;   // smooth_photoion_rate planes
+  DevBuf<ChemCounters> chem_ctr;
+  hipEvent_t ev_c0 = nullptr, ev_c1 = nullptr;
+  argot_chem_stats chem_stats{};
   HostTree tree;
   argot_step_stats stats{};
   // step graph
@@ -332,12 +337,15 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   if (!g.ev0) HIP_OK(hipEventCreate(&g.ev0));
   if (!g.ev1) HIP_OK(hipEventCreate(&g.ev1));
   if (!g.ev_mid) HIP_OK(hipEventCreate(&g.ev_mid));
+  if (!g.ev_c0) HIP_OK(hipEventCreate(&g.ev_c0));
+  if (!g.ev_c1) HIP_OK(hipEventCreate(&g.ev_c1));
   this_cuda->strm[0] = g.stream;
   this_cuda->diffuse_strm[0] = g.stream;
   g.mesh = reinterpret_cast<unsigned char*>(cuda_mem->mesh_dev);
   upload_tables(this_run);
   if (t_err) return;
   HIP_OK(g.ctr.reserve(1));
+  HIP_OK(g.chem_ctr.reserve(1));
   g.ready = true;
 }
 
@@ -506,15 +514,11 @@ void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_me
   HIP_OK(hipStreamSynchronize(g.stream));
 }
 
-void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
-                         struct run_param* this_run, struct mpi_param* this_mpi,
-                         struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
-                         float dtime) {
-  (void)mesh;
-  (void)this_mpi;
-  (void)this_cuda;
-  (void)dtime;
-  t_err = ARGOT_OK;
+// One point-source pass of step_radiation_tree (calc_radiation_transfer
+// with zero_out_photoion_rate_cuda before it): rates into
+// mesh_dev[].prev_chem, counters into g.stats.
+static void point_source_pass(struct radiation_src* src, struct run_param* this_run,
+                              struct cuda_mem_space* cuda_mem) {
   const double t_start = now_ms();
   if (!check_ready(cuda_mem, this_run)) return;
   if (this_run->rank_x || this_run->rank_y || this_run->rank_z ||
@@ -665,6 +669,150 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   this_run->nray = hc.nray;
   this_run->nseg = hc.nray;
 }
+
+static ChemParams chem_params(const run_param* r) {
+  ChemParams p{};
+  p.lunit = r->lunit;
+  p.munit = r->munit;
+  p.tunit = r->tunit;
+  p.uenetok = r->uenetok;
+  p.denstonh = r->denstonh;
+  p.helium_fact = g_helium_fact;
+  p.anow = r->anow;
+  p.znow = r->znow;
+  return p;
+}
+
+// The chemistry kernel over every cell, timed with HIP events; the maxima
+// of the relative changes as the reference reduces them (max over blocks
+// starting from 0, then MAX(FLT_MIN, .)).
+static void chem_step(const run_param* r, float dtime, float* diff_chem, float* diff_uene) {
+  if (!(r->tunit > 0.0) || !(r->uenetok > 0.0) || !(r->munit > 0.0) || !(r->lunit > 0.0))
+    return set_err(ARGOT_ERR_ARG);
+  HIP_OK(hipEventRecord(g.ev_c0, g.stream));
+  HIP_OK(launch_chem_step(g.mesh, g.ncell, chem_params(r), dtime, g.chem_ctr.p, g.stream));
+  HIP_OK(hipEventRecord(g.ev_c1, g.stream));
+  ChemCounters hc;
+  HIP_OK(hipMemcpyAsync(&hc, g.chem_ctr.p, sizeof hc, hipMemcpyDeviceToHost, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, g.ev_c0, g.ev_c1));
+  float dc, du;
+  memcpy(&dc, &hc.diff_chem, sizeof dc);
+  memcpy(&du, &hc.diff_uene, sizeof du);
+  const float fmin_norm = FLT_MIN;
+  if (diff_chem) *diff_chem = fmin_norm > dc ? fmin_norm : dc;
+  if (diff_uene) *diff_uene = fmin_norm > du ? fmin_norm : du;
+  g.chem_stats.ncell = (uint64_t)g.ncell;
+  g.chem_stats.nsub = hc.nsub;
+  g.chem_stats.niter = hc.niter;
+  g.chem_stats.ms_kernel = ms;
+}
+
+void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
+                         struct run_param* this_run, struct mpi_param* this_mpi,
+                         struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                         float dtime) {
+  (void)mesh;
+  (void)this_mpi;
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  const double t_start = now_ms();
+  g.stats.iterations = 1;
+  g.stats.ms_chem = 0.0;
+  point_source_pass(src, this_run, cuda_mem);
+  if (t_err || g_iterations <= 0) return;
+  // src/step_radiation_tree.c:666-723: smooth, chemistry, and another pass
+  // while the chemistry or the energy still moved by more than 1e-3.
+  uint64_t nray = g.stats.nray, ncross = g.stats.ncross;
+  double ms_kernel = g.stats.ms_kernel, ms_chem = 0.0;
+  int iter = 1;
+  for (;;) {
+    HIP_OK(g.smooth_in.reserve(6 * (size_t)g.ncell));
+    HIP_OK(g.smooth_out.reserve(6 * (size_t)g.ncell));
+    HIP_OK(hipEventRecord(g.ev_c0, g.stream));
+    HIP_OK(launch_smooth_rates(g.mesh, g.nx, g.ny, g.nz, g.smooth_in.p, g.smooth_out.p,
+                               g.stream));
+    HIP_OK(hipEventRecord(g.ev_c1, g.stream));
+    HIP_OK(hipEventSynchronize(g.ev_c1));
+    float ms_s = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms_s, g.ev_c0, g.ev_c1));
+    float dc = 0.0f, du = 0.0f;
+    chem_step(this_run, dtime, &dc, &du);
+    if (t_err) return;
+    ms_chem += ms_s + g.chem_stats.ms_kernel;
+    if (!((dc > 1.0e-3 || du > 1.0e-3) && iter < g_iterations)) break;
+    point_source_pass(src, this_run, cuda_mem);
+    if (t_err) return;
+    nray += g.stats.nray;
+    ncross += g.stats.ncross;
+    ms_kernel += g.stats.ms_kernel;
+    iter++;
+  }
+  HIP_OK(launch_chem_update(g.mesh, g.ncell, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+  g.stats.iterations = (uint64_t)iter;
+  g.stats.nray = nray;
+  g.stats.ncross = ncross;
+  g.stats.ms_kernel = ms_kernel;
+  g.stats.ms_chem = ms_chem;
+  g.stats.ms_total = now_ms() - t_start;
+}
+
+void step_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                        struct run_param* this_run, float* diff_chem, float* diff_uene,
+                        float dtime) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  chem_step(this_run, dtime, diff_chem, diff_uene);
+}
+
+void step_chemistry(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                    struct run_param* this_run, float dtime, float* max_diff_chem,
+                    float* max_diff_uene) {
+  step_chemistry_gpu(cuda_mem, this_cuda, this_run, max_diff_chem, max_diff_uene, dtime);
+}
+
+double calc_dtime_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                      struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return 0.0;
+  ChemCounters hc;
+  hipError_t e = launch_chem_dtime(g.mesh, g.ncell, chem_params(this_run), g.chem_ctr.p, g.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&hc, g.chem_ctr.p, sizeof hc, hipMemcpyDeviceToHost, g.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+  if (e != hipSuccess) {
+    set_hip_err(e, __LINE__);
+    return 0.0;
+  }
+  float dt;
+  memcpy(&dt, &hc.dt_min, sizeof dt);
+  return (double)dt;
+}
+
+double calc_timestep_chem(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                          struct run_param* this_run) {
+  const double dt = calc_dtime_gpu(cuda_mem, this_cuda, this_run);
+  return t_err ? 0.0 : dt / this_run->tunit;
+}
+
+void update_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                          struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  HIP_OK(launch_chem_update(g.mesh, g.ncell, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void argot_hip_chem_stats(struct argot_chem_stats* out) {
+  if (out) *out = g.chem_stats;
+}
+
+void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
 // streaming kernel; returns the best GB/s (read + write bytes / time).

@@ -17,6 +17,28 @@ import numpy as np
 MPC_CM = 3.086e24                  # constants.h:14
 LUNIT_CM = 6.6e-3 * MPC_CM         # 6.6 kpc = 2.03676e22 cm
 DENSTONH = 1.0e-3                  # n_H [cm^-3] per code density unit
+MPROTON_G = 1.672623e-24           # constants.h:15
+KBOLTZ = 1.38066e-16               # constants.h:19
+YEAR_S = 3.156e7                   # constants.h:24
+XHYDROGEN, YHELIUM = 0.755, 0.245  # prim_chem/chemistry.h:23-24
+GAMM1_MONOATOMIC = 0.66666666666666666   # prim_chem/chemistry.h:18
+TUNIT_S = 1.0e8 * YEAR_S           # 100 Myr
+
+
+def chem_units(lunit=LUNIT_CM, denstonh=DENSTONH, tunit=TUNIT_S):
+    """munit, tunit and uenetok of the reference's comparison set-up
+    (src/test_IC/cmpproj_test2_IC.c:72-79): code density 1 == n_H = denstonh,
+    u * uenetok * wmol = T [K]."""
+    munit = denstonh * lunit * lunit * lunit * MPROTON_G / XHYDROGEN
+    uenetok = GAMM1_MONOATOMIC * MPROTON_G / KBOLTZ * (lunit * lunit) / (tunit * tunit)
+    return {"lunit": lunit, "munit": munit, "tunit": tunit, "uenetok": uenetok,
+            "denstonh": denstonh}
+
+
+def wmol(felec):
+    """Mean molecular weight 1/((1 + x_e) X + Y/4) (chemistry.h:69), float32."""
+    fe = np.asarray(felec, np.float64)
+    return (1.0 / ((1.0 + fe) * XHYDROGEN + 0.25 * YHELIUM)).astype(np.float32)
 
 
 def helium_fact(y_he):

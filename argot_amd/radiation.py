@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import abi
+from . import abi, ic
 from ._lib import check, lib
 
 RATE_FIELDS = ("GammaHI", "HeatHI", "GammaHeI", "GammaHeII", "HeatHeI", "HeatHeII")
@@ -61,7 +61,10 @@ def run_param_for(w, freq, cs):
     r.nnode_x = r.nnode_y = r.nnode_z = 1
     r.lunit = w.lunit
     r.denstonh = w.denstonh
+    u = ic.chem_units(w.lunit, w.denstonh)
+    r.munit, r.tunit, r.uenetok = u["munit"], u["tunit"], u["uenetok"]
     r.anow = 1.0
+    r.znow = 0.0
     r.freq = freq
     C.memmove(C.addressof(r.csect), C.addressof(cs), C.sizeof(cs))
     r.nspecies = 5
@@ -81,7 +84,8 @@ def run_param_for(w, freq, cs):
 
 
 def fill_mesh(w):
-    """struct fluid_mesh[n^3] holding the workload's density and fractions."""
+    """struct fluid_mesh[n^3] holding the workload's density, fractions and
+    temperature (uene = prev_uene = T / (uenetok wmol), felec from the ions)."""
     ncell = w.n ** 3
     mesh = abi.mesh_array(ncell)
     v = abi.mesh_view(mesh)
@@ -92,7 +96,15 @@ def fill_mesh(w):
     chem["fHeI"] = w.fHeI.reshape(-1)
     chem["fHeII"] = w.fHeII.reshape(-1)
     chem["fHeIII"] = w.fHeIII.reshape(-1)
+    fe = (chem["fHII"].astype(np.float64) + w.helium_fact *
+          (chem["fHeII"].astype(np.float64) + 2.0 * chem["fHeIII"].astype(np.float64)))
+    chem["felec"] = fe.astype(np.float32)
+    chem["wmol"] = ic.wmol(chem["felec"])
     v["prev_chem"] = chem
+    uenetok = ic.chem_units(w.lunit, w.denstonh)["uenetok"]
+    u = (w.temperature / (uenetok * chem["wmol"].astype(np.float64))).astype(np.float32)
+    v["uene"] = u
+    v["prev_uene"] = u
     return mesh
 
 
@@ -115,15 +127,64 @@ class PointSourceStep:
         L.send_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
         check("send_mesh_data")
 
-    def step(self):
+    def step(self, dtime=0.0, iterations=0):
+        """step_radiation_tree: one point-source pass (iterations=0), or the
+        reference's transfer/smoothing/chemistry loop of at most
+        `iterations` passes over dtime (code units)."""
         L = lib()
         L.argot_hip_set_helium_fact(self.w.helium_fact)
         L.argot_hip_set_tau_mode(self.tau_mode)
         L.argot_hip_set_rate_mode(self.rate_mode)
-        L.step_radiation_tree(self.mesh, self.src, C.byref(self.run), None,
-                              C.byref(self.cmem), C.byref(self.cpar), 0.0)
+        L.argot_hip_set_iterations(iterations)
+        try:
+            L.step_radiation_tree(self.mesh, self.src, C.byref(self.run), None,
+                                  C.byref(self.cmem), C.byref(self.cpar), dtime)
+        finally:
+            L.argot_hip_set_iterations(0)
         check("step_radiation_tree")
         return self.stats()
+
+    def chemistry(self, dtime):
+        """step_chemistry over dtime (code units): (max_diff_chem, max_diff_uene)."""
+        L = lib()
+        dc, du = C.c_float(0.0), C.c_float(0.0)
+        L.argot_hip_set_helium_fact(self.w.helium_fact)
+        L.step_chemistry(C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run), dtime,
+                         C.byref(dc), C.byref(du))
+        check("step_chemistry")
+        return np.float32(dc.value), np.float32(du.value)
+
+    def chem_stats(self):
+        st = abi.ChemStats()
+        lib().argot_hip_chem_stats(C.byref(st))
+        return {k: getattr(st, k) for k, _ in abi.ChemStats._fields_}
+
+    def chem_timestep(self):
+        """(calc_dtime_gpu [s], calc_timestep_chem [code units])."""
+        L = lib()
+        L.argot_hip_set_helium_fact(self.w.helium_fact)
+        dt = L.calc_dtime_gpu(C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("calc_dtime_gpu")
+        dtc = L.calc_timestep_chem(C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("calc_timestep_chem")
+        return dt, dtc
+
+    def update_chemistry(self):
+        lib().update_chemistry_gpu(C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("update_chemistry_gpu")
+
+    def mesh_bytes(self):
+        """The device mesh read back (recv_mesh_data), as bytes of struct
+        fluid_mesh[] (a copy)."""
+        L = lib()
+        L.recv_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("recv_mesh_data")
+        return np.frombuffer(self.mesh, dtype=np.uint8).copy()
+
+    def send_mesh(self):
+        lib().send_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar),
+                             C.byref(self.run))
+        check("send_mesh_data")
 
     def smooth(self):
         """smooth_photoion_rate on the device rates; the host mesh receives

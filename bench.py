@@ -108,6 +108,27 @@ def surrogate_leg(step, plain_seg_s, nray):
     return out
 
 
+def chem_leg(step, dtime=1.0e-3):
+    """Chemistry of the whole mesh over dtime (code units) after the pass,
+    one thread per cell (step_chemistry); run twice, the second timed."""
+    from argot_amd import ic
+    step.chemistry(np.float32(dtime))
+    t0 = time.perf_counter()
+    dc, du = step.chemistry(np.float32(dtime))
+    ms = 1000.0 * (time.perf_counter() - t0)
+    cs = step.chem_stats()
+    ncell = int(cs["ncell"])
+    out = {"workload": f"{step.w.name} mesh after the point-source pass",
+           "dtime_code": dtime, "dtime_yr": dtime * step.run.tunit / ic.YEAR_S,
+           "ms_kernel": cs["ms_kernel"], "ms_call": ms,
+           "cells_per_s": ncell / (cs["ms_kernel"] * 1e-3),
+           "substeps": int(cs["nsub"]), "bisections": int(cs["niter"]),
+           "substeps_per_s": cs["nsub"] / (cs["ms_kernel"] * 1e-3),
+           "max_diff_chem": float(dc), "max_diff_uene": float(du)}
+    log(f"chemistry: {out}")
+    return out
+
+
 def art_leg():
     """ART diffuse transfer, 128^3 lognormal mesh, 768 HEALPix directions."""
     from argot_amd import art as A
@@ -141,7 +162,8 @@ def main():
     ap.add_argument("--tile", default=None, help="fixed target tile, e.g. 4x4x4 (skips tuning)")
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the surrogate and ART legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the surrogate, chemistry and ART legs")
     ap.add_argument("--art-only", action="store_true",
                     help="run only the ART leg and print its JSON (profiling)")
     args = ap.parse_args()
@@ -236,6 +258,7 @@ def main():
     }
     if not args.no_extras:
         out["surrogate"] = surrogate_leg(step, seg_s, nray)
+        out["chemistry"] = chem_leg(step)
         out["art"] = art_leg()
         log(f"art: {out['art']}")
     if not args.no_cpu_baseline:

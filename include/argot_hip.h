@@ -240,6 +240,56 @@ void zero_out_photoion_rate_cuda(struct cuda_mem_space *cuda_mem,
 void smooth_photoion_rate(struct fluid_mesh *mesh, struct run_param *this_run,
                           struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
                           struct mpi_param *this_mpi);
+
+/* Chemistry on the device (helium on, H2 off, heating/cooling on, case-B
+ * recombination), one thread per cell.  Each cell starts from
+ * mesh_dev[].prev_chem (fractions and the photoionization/heating rates)
+ * and prev_uene and integrates the network and the energy over dtime (code
+ * units); the results go to mesh_dev[].chem and .uene, .duene := 0.
+ * Units come from this_run (lunit, munit, tunit, uenetok, denstonh, anow,
+ * znow); the helium fraction from argot_hip_set_helium_fact. */
+/* replaces step_chemistry_gpu, src/chemistry.cu:125-222: *diff_chem and
+ * *diff_uene receive max(FLT_MIN, the largest relative change over cells)
+ * of (GammaHI, felec) against the previous mesh_dev[].chem, and of uene. */
+void step_chemistry_gpu(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                        struct run_param *this_run, float *diff_chem, float *diff_uene,
+                        float dtime);
+/* replaces step_chemistry (GPU variant), src/prototype.h:92 and
+ * src/step_chemistry.c:53-88: step_chemistry_gpu on one rank (the
+ * reduction over ranks is the identity). */
+void step_chemistry(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                    struct run_param *this_run, float dtime, float *max_diff_chem,
+                    float *max_diff_uene);
+/* replaces calc_dtime_gpu, src/chemistry.cu:56-120: the smallest safe
+ * chemistry step over cells, in seconds. */
+double calc_dtime_gpu(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                      struct run_param *this_run);
+/* replaces calc_timestep_chem (GPU variant), src/prototype.h:90 and
+ * src/calc_timestep.c:21-32: calc_dtime_gpu / tunit on one rank. */
+double calc_timestep_chem(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                          struct run_param *this_run);
+/* replaces update_chemistry_gpu, src/prototype.h:118 and
+ * src/chemistry.cu:32-54: prev_chem := chem, prev_uene := uene,
+ * eneg := uene dens + |mom|^2 / (2 dens). */
+void update_chemistry_gpu(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
+                          struct run_param *this_run);
+
+/* Counters of the last step_chemistry(_gpu) call. */
+struct argot_chem_stats {
+    uint64_t ncell;          /* cells */
+    uint64_t nsub;           /* network sub-steps, summed over cells */
+    uint64_t niter;          /* energy bisections, summed over cells */
+    double ms_kernel;        /* chemistry kernel, HIP events */
+};
+void argot_hip_chem_stats(struct argot_chem_stats *out);
+
+/* Iterations of step_radiation_tree.  0 (the default): one point-source
+ * pass, rates left in mesh_dev[].prev_chem.  n > 0: the reference's loop
+ * (src/step_radiation_tree.c:666-723) of at most n passes of transfer,
+ * smooth_photoion_rate and step_chemistry(dtime) while the largest relative
+ * change of chemistry or energy exceeds 1e-3, then update_chemistry_gpu;
+ * the reference runs n = 5. */
+void argot_hip_set_iterations(int n);
 /* replaces merge_photoion_data, src/merge_photoion_data.cu:16-83:
  * mesh_dev[].prev_chem += host mesh[].prev_chem rates. */
 void merge_photoion_data(struct fluid_mesh *mesh,
@@ -380,6 +430,8 @@ struct argot_step_stats {
     double ms_kernel;        /* fused kernel(s), HIP events */
     double ms_total;         /* whole call, wall clock */
     uint64_t nsurrogate;     /* rays whose rates came from the MLP surrogate */
+    uint64_t iterations;     /* transfer passes (1 unless argot_hip_set_iterations) */
+    double ms_chem;          /* smoothing + chemistry kernels over all passes */
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
 

@@ -222,6 +222,43 @@ void orc_art_gh_tot(long ncell, const float *length, const float *I, const float
 long orc_art_run(const orc_grid *g, int nside, int heatcool, const float *dens,
                  const float *chem, double uenetok, float *rates);
 
+/* ---- chemistry (chem.c) ---- */
+
+/* struct prim_chem (13 floats). */
+typedef struct {
+    float wmol, felec, fHI, fHII, fHeI, fHeII, fHeIII;
+    float GammaHI, HeatHI, GammaHeI, GammaHeII, HeatHeI, HeatHeII;
+} orc_chem;
+
+/* Units and cosmology of struct run_param, plus the helium number fraction. */
+typedef struct {
+    double lunit, munit, tunit, uenetok, denstonh, helium_fact;
+    float anow, znow;
+} orc_chem_params;
+
+/* k01..k06 (case B) at temperature t [K]. */
+void orc_chem_k(double t, double *k);
+/* creh2 crehe2 crexihe2 crehe3 cioh1 ciohe1 ciohe2 cexh1 cexhe1 cexhe2 brems compt. */
+void orc_chem_cooling(double t, float zred, double *o);
+/* calc_heatcool_rate_dev: heat - cool [erg cm^-3 s^-1]. */
+double orc_chem_heatcool(const orc_chem *c, float zred, double nH, double T, double helium_fact);
+/* advance_heatcool_dev: *uene over dtime (code units); returns the bisections. */
+int orc_chem_advance_heatcool(float dens, float *uene, const orc_chem *c,
+                              const orc_chem_params *p, float dtime);
+/* calc_dtime_dev: safe step [s] (mesh_uene: the mesh's current energy). */
+float orc_chem_dtime(float dens, float mesh_uene, const orc_chem *c, const orc_chem_params *p);
+/* advance_reaction_and_heatcool_dev on one cell. */
+void orc_chem_advance(float dens, float mesh_uene, float *uene, orc_chem *c,
+                      const orc_chem_params *p, float dtime, int *nsub, int *niter);
+/* The per-cell kernels of chemistry.cu over an AoS struct fluid_mesh array:
+ * the step (maxima of the relative changes, work counts), the minimum safe
+ * step over cells, and update_chemistry. */
+void orc_chem_step_mesh(unsigned char *mesh, int64_t ncell, const orc_chem_params *p,
+                        float dtime, float *diff_chem, float *diff_uene, uint64_t *nsub,
+                        uint64_t *niter);
+float orc_chem_dtime_mesh(const unsigned char *mesh, int64_t ncell, const orc_chem_params *p);
+void orc_chem_update_mesh(unsigned char *mesh, int64_t ncell);
+
 #ifdef __cplusplus
 }
 #endif

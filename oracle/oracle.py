@@ -376,3 +376,136 @@ def smooth_rates(rates, shape):
     out = np.zeros_like(r)
     L.orc_smooth_rates(shape[0], shape[1], shape[2], r, out)
     return out.reshape(-1, 6)
+
+
+# ---------------------------------------------------------------- chemistry
+
+class ChemParams(C.Structure):
+    """orc_chem_params: run_param's units and cosmology + helium fraction."""
+    _fields_ = [("lunit", C.c_double), ("munit", C.c_double), ("tunit", C.c_double),
+                ("uenetok", C.c_double), ("denstonh", C.c_double),
+                ("helium_fact", C.c_double), ("anow", C.c_float), ("znow", C.c_float)]
+
+
+CHEM_FIELDS = ("wmol", "felec", "fHI", "fHII", "fHeI", "fHeII", "fHeIII",
+               "GammaHI", "HeatHI", "GammaHeI", "GammaHeII", "HeatHeI", "HeatHeII")
+COOLING_NAMES = ("creh2", "crehe2", "crexihe2", "crehe3", "cioh1", "ciohe1", "ciohe2",
+                 "cexh1", "cexhe1", "cexhe2", "brems", "compt")
+_chem_bound = False
+
+
+def _chem_lib():
+    global _chem_bound
+    L = lib()
+    if not _chem_bound:
+        P = C.POINTER
+        cp = P(ChemParams)
+        L.orc_chem_k.argtypes = [C.c_double, f64p]
+        L.orc_chem_cooling.argtypes = [C.c_double, C.c_float, f64p]
+        L.orc_chem_heatcool.argtypes = [f32p, C.c_float, C.c_double, C.c_double, C.c_double]
+        L.orc_chem_heatcool.restype = C.c_double
+        L.orc_chem_advance_heatcool.argtypes = [C.c_float, P(C.c_float), f32p, cp, C.c_float]
+        L.orc_chem_dtime.argtypes = [C.c_float, C.c_float, f32p, cp]
+        L.orc_chem_dtime.restype = C.c_float
+        L.orc_chem_advance.argtypes = [C.c_float, C.c_float, P(C.c_float), f32p, cp, C.c_float,
+                                       P(C.c_int), P(C.c_int)]
+        u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+        L.orc_chem_step_mesh.argtypes = [u8p, C.c_int64, cp, C.c_float, P(C.c_float),
+                                         P(C.c_float), P(C.c_uint64), P(C.c_uint64)]
+        L.orc_chem_dtime_mesh.argtypes = [u8p, C.c_int64, cp]
+        L.orc_chem_dtime_mesh.restype = C.c_float
+        L.orc_chem_update_mesh.argtypes = [u8p, C.c_int64]
+        _chem_bound = True
+    return L
+
+
+def chem_k(t):
+    k = np.zeros(6)
+    _chem_lib().orc_chem_k(float(t), k)
+    return k
+
+
+def chem_cooling(t, zred=0.0):
+    o = np.zeros(12)
+    _chem_lib().orc_chem_cooling(float(t), zred, o)
+    return o
+
+
+def chem_heatcool(chem13, zred, nH, T, helium_fact):
+    return _chem_lib().orc_chem_heatcool(np.ascontiguousarray(chem13, np.float32), zred, nH, T,
+                                         helium_fact)
+
+
+def chem_advance_heatcool(dens, uene, chem13, p, dtime):
+    """(new uene, bisection count)."""
+    u = C.c_float(uene)
+    n = _chem_lib().orc_chem_advance_heatcool(dens, C.byref(u),
+                                              np.ascontiguousarray(chem13, np.float32),
+                                              C.byref(p), dtime)
+    return np.float32(u.value), n
+
+
+def chem_dtime(dens, mesh_uene, chem13, p):
+    return np.float32(_chem_lib().orc_chem_dtime(dens, mesh_uene,
+                                                 np.ascontiguousarray(chem13, np.float32),
+                                                 C.byref(p)))
+
+
+def chem_advance(dens, mesh_uene, uene, chem13, p, dtime):
+    """One cell: (new chem13, new uene, sub-steps, bisections)."""
+    c = np.array(chem13, np.float32)
+    u = C.c_float(uene)
+    ns, ni = C.c_int(0), C.c_int(0)
+    _chem_lib().orc_chem_advance(dens, mesh_uene, C.byref(u), c, C.byref(p), dtime,
+                                 C.byref(ns), C.byref(ni))
+    return c, np.float32(u.value), ns.value, ni.value
+
+
+def chem_step_mesh(mesh_bytes, p, dtime):
+    """step_chemistry_gpu on a copy-free uint8 view of struct fluid_mesh[]
+    (modified in place): (diff_chem, diff_uene, nsub, niter)."""
+    m = mesh_bytes.reshape(-1)
+    dc, du = C.c_float(0), C.c_float(0)
+    ns, ni = C.c_uint64(0), C.c_uint64(0)
+    _chem_lib().orc_chem_step_mesh(m, m.size // 156, C.byref(p), dtime, C.byref(dc),
+                                   C.byref(du), C.byref(ns), C.byref(ni))
+    return np.float32(dc.value), np.float32(du.value), ns.value, ni.value
+
+
+def chem_dtime_mesh(mesh_bytes, p):
+    m = mesh_bytes.reshape(-1)
+    return np.float32(_chem_lib().orc_chem_dtime_mesh(m, m.size // 156, C.byref(p)))
+
+
+def chem_update_mesh(mesh_bytes):
+    m = mesh_bytes.reshape(-1)
+    _chem_lib().orc_chem_update_mesh(m, m.size // 156)
+
+
+REF_CHEM = os.path.join(REF_DIR, "libref_chem.so")
+_ref_chem = None
+
+
+def ref_chem():
+    """The reference's own CPU chemistry (oracle/_ref/libref_chem.so), or None
+    when it is not built (it needs the reference sources)."""
+    global _ref_chem
+    if _ref_chem is not None:
+        return _ref_chem
+    if not os.path.exists(REF_CHEM):
+        return None
+    L = C.CDLL(REF_CHEM)
+    P = C.POINTER
+    L.ref_chem_helium_fact.restype = C.c_double
+    L.ref_chem_k.argtypes = [C.c_double, f64p]
+    L.ref_chem_cooling.argtypes = [C.c_double, C.c_float, f64p]
+    L.ref_chem_heatcool.argtypes = [f32p, C.c_float, C.c_double, C.c_double]
+    L.ref_chem_heatcool.restype = C.c_double
+    L.ref_chem_advance_heatcool.argtypes = [C.c_float, C.c_float, P(C.c_float), f32p, f64p,
+                                            C.c_float, C.c_float, C.c_float]
+    L.ref_chem_dtime.argtypes = [C.c_float, C.c_float, f32p, f64p, C.c_float, C.c_float]
+    L.ref_chem_dtime.restype = C.c_double
+    L.ref_chem_advance.argtypes = [C.c_float, C.c_float, P(C.c_float), f32p, f64p, C.c_float,
+                                   C.c_float, C.c_float]
+    _ref_chem = L
+    return L

@@ -1,0 +1,235 @@
+"""The chemistry oracle (oracle/chem.c, the GPU chemistry of chemistry.cu
+restated in C) pinned three ways:
+
+* against the reference's own CPU chemistry compiled from its sources
+  (oracle/_ref/libref_chem.so): the rate fits, the cooling functions,
+  heat - cool and the energy update are the same code there, so they must
+  agree bit for bit; the network step and the safe step use double
+  densities there and float ones on the GPU, so they must agree closely
+  (a dropped term or a wrong sign moves them by far more);
+* against properties of the scheme that hold on any machine: element
+  fractions renormalised, the electron fraction formed from the ions, no
+  change over a zero step, the energy update solving its implicit equation,
+  photoionization equilibrium after a long step;
+* at the reference's branch points (k03/k05 at T = 0.8 eV, k04 at 500.3 K,
+  crehe2 at 980 K) on both sides.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from argot_amd import ic
+
+REF = O.ref_chem()
+needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref/libref_chem.so not built")
+
+KTOEV = 8.61623e-5
+
+
+def units(helium_fact):
+    u = ic.chem_units()
+    return O.ChemParams(u["lunit"], u["munit"], u["tunit"], u["uenetok"], u["denstonh"],
+                        helium_fact, 1.0, 0.0), np.array(
+        [u["lunit"], u["munit"], u["tunit"], u["uenetok"], u["denstonh"]])
+
+
+def random_state(rng, hf, gamma=None):
+    """A cell state: fractions, felec from the ions, rates of one spectrum."""
+    x = 10 ** rng.uniform(-5, 0)
+    a, b = rng.uniform(0, 1, 2)
+    c = np.zeros(13, np.float32)
+    c[2], c[3] = 1 - x, x
+    c[4], c[5], c[6] = 1 - a * x, a * x * (1 - b), a * x * b
+    c[1] = c[3] + hf * (c[5] + 2.0 * c[6])
+    c[0] = ic.wmol(c[1])
+    G = 10 ** rng.uniform(-16, -11) if gamma is None else gamma
+    c[7], c[8] = G, G * 3e-12
+    c[9], c[10] = 0.6 * G, 0.01 * G
+    c[11], c[12] = 2e-12 * G, 1e-13 * G
+    return c
+
+
+def temps():
+    t = list(np.logspace(1.0, 9.0, 300))
+    for edge in (500.3, 980.0, 0.8 / KTOEV):
+        t += [np.nextafter(edge, 0), edge, np.nextafter(edge, np.inf)]
+    return t
+
+
+@needs_ref
+def test_rate_fits_match_the_reference_build():
+    for t in temps():
+        k, r = O.chem_k(t), np.zeros(6)
+        REF.ref_chem_k(t, r)
+        assert np.array_equal(k.view(np.uint64), r.view(np.uint64)), t
+
+
+@needs_ref
+@pytest.mark.parametrize("zred", [0.0, 3.5])
+def test_cooling_functions_match_the_reference_build(zred):
+    for t in temps():
+        o, r = O.chem_cooling(t, zred), np.zeros(12)
+        REF.ref_chem_cooling(t, zred, r)
+        assert np.array_equal(o.view(np.uint64), r.view(np.uint64)), (t, O.COOLING_NAMES)
+
+
+@needs_ref
+def test_heat_minus_cool_matches_the_reference_build():
+    rng = np.random.default_rng(7)
+    hf = REF.ref_chem_helium_fact()
+    for _ in range(400):
+        c = random_state(rng, hf)
+        nH, T = 10 ** rng.uniform(-5, 1), 10 ** rng.uniform(1, 8.5)
+        z = float(rng.choice([0.0, 2.0]))
+        a = O.chem_heatcool(c, z, nH, T, hf)
+        b = REF.ref_chem_heatcool(c, z, nH, T)
+        assert a == b or (np.isnan(a) and np.isnan(b)), (a, b)
+
+
+@needs_ref
+def test_energy_update_matches_the_reference_build():
+    rng = np.random.default_rng(8)
+    hf = REF.ref_chem_helium_fact()
+    p, uv = units(hf)
+    for i in range(300):
+        c = random_state(rng, hf)
+        T = 10 ** rng.uniform(0.5, 9.0)        # includes both clamps (10 K, 1e8 K)
+        u0 = np.float32(T / (p.uenetok * c[0]))
+        dens = np.float32(10 ** rng.uniform(-2, 3))
+        dt = np.float32(10 ** rng.uniform(-6, -1))
+        u1, _ = O.chem_advance_heatcool(dens, u0, c, p, dt)
+        ur = C.c_float(u0)
+        REF.ref_chem_advance_heatcool(dens, u0, C.byref(ur), c, uv, 1.0, 0.0, dt)
+        assert u1 == np.float32(ur.value), (i, u1, ur.value)
+
+
+@needs_ref
+def test_network_step_tracks_the_reference_cpu_path():
+    """GPU arithmetic (float densities) against the reference's CPU code
+    (double densities) on the same cells: the sub-step counts, fractions and
+    energies agree to a few 1e-4; a missing source or sink term would move
+    them by orders of magnitude more."""
+    rng = np.random.default_rng(9)
+    hf = REF.ref_chem_helium_fact()
+    p, uv = units(hf)
+    worst = np.zeros(8)
+    for _ in range(120):
+        c = random_state(rng, hf)
+        T = 10 ** rng.uniform(2.0, 5.0)
+        u0 = np.float32(T / (p.uenetok * c[0]))
+        dens = np.float32(10 ** rng.uniform(-1, 2))
+        dt = np.float32(10 ** rng.uniform(-4, -2))
+        co, uo, _, _ = O.chem_advance(dens, u0, u0, c, p, dt)
+        cr = c.copy()
+        ur = C.c_float(u0)
+        REF.ref_chem_advance(dens, u0, C.byref(ur), cr, uv, 1.0, 0.0, dt)
+        rel = np.abs(co[1:7] - cr[1:7]) / np.maximum(np.abs(cr[1:7]), 1e-6)
+        worst[:6] = np.maximum(worst[:6], rel)
+        worst[6] = max(worst[6], abs(uo - ur.value) / ur.value)
+    assert worst[:6].max() < 3e-3, worst
+    assert worst[6] < 1e-3, worst
+
+
+@needs_ref
+def test_safe_step_tracks_the_reference_cpu_path():
+    rng = np.random.default_rng(10)
+    hf = REF.ref_chem_helium_fact()
+    p, uv = units(hf)
+    for _ in range(300):
+        c = random_state(rng, hf)
+        T = 10 ** rng.uniform(1.5, 7.0)
+        u0 = np.float32(T / (p.uenetok * c[0]))
+        dens = np.float32(10 ** rng.uniform(-2, 3))
+        a = O.chem_dtime(dens, u0, c, p)
+        b = REF.ref_chem_dtime(dens, u0, c, uv, 1.0, 0.0)
+        assert abs(a - b) <= 1e-3 * b, (a, b)
+
+
+def test_zero_step_changes_nothing():
+    rng = np.random.default_rng(11)
+    hf = ic.helium_fact(0.24)
+    p, _ = units(hf)
+    for _ in range(20):
+        c = random_state(rng, hf)
+        u0 = np.float32(1e4 / (p.uenetok * c[0]))
+        co, uo, ns, _ = O.chem_advance(np.float32(1.0), u0, u0, c, p, np.float32(0.0))
+        assert ns == 0 and uo == u0 and np.array_equal(co, c)
+
+
+def test_fractions_and_electrons_are_consistent():
+    rng = np.random.default_rng(12)
+    hf = ic.helium_fact(0.24)
+    p, _ = units(hf)
+    for _ in range(60):
+        c = random_state(rng, hf)
+        u0 = np.float32(10 ** rng.uniform(2, 5) / (p.uenetok * c[0]))
+        co, _, ns, _ = O.chem_advance(np.float32(10 ** rng.uniform(-1, 2)), u0, u0, c, p,
+                                      np.float32(1e-3))
+        assert ns >= 1
+        assert abs(co[2] + co[3] - 1.0) < 4e-7
+        assert abs(co[4] + co[5] + co[6] - 1.0) < 4e-7
+        fe = np.float64(co[3]) + (hf * (np.float64(co[5]) + 2.0 * np.float64(co[6])) + 1e-31)
+        assert co[1] == np.float32(fe)
+        assert np.all(co[2:7] >= 0.0)
+
+
+def test_energy_update_solves_its_implicit_equation():
+    """f(u) = u - u0 - dt Lambda((u0 + u)/2) changes sign within the final
+    bisection interval (1e-5 u) around the result, with Lambda the
+    (reference-pinned) heat - cool per unit mass.  The reference searches
+    for the bracket with Lambda at the bracket end rather than at the
+    midpoint, so where the cooling is stiff the bracket can miss the root of
+    f and the bisection stops at the bracket end: most cells, not all."""
+    rng = np.random.default_rng(13)
+    hf = ic.helium_fact(0.24)
+    p, _ = units(hf)
+    eunit = p.lunit ** 2 / p.tunit ** 2
+    conv = p.tunit / eunit * p.lunit ** 3 / p.munit
+    checked = bracketed = 0
+    for _ in range(200):
+        c = random_state(rng, hf)
+        T = 10 ** rng.uniform(3.0, 6.0)
+        u0 = np.float32(T / (p.uenetok * c[0]))
+        dens = np.float32(10 ** rng.uniform(-1, 1))
+        dt = np.float32(10 ** rng.uniform(-5, -3))
+        u1, n = O.chem_advance_heatcool(dens, u0, c, p, dt)
+        if n >= 32:
+            continue
+        nH = float(dens) * p.denstonh
+        to_T = p.uenetok * float(np.float32(c[0]))
+
+        def f(u):
+            lam = O.chem_heatcool(c, 0.0, nH, 0.5 * (float(u0) + u) * to_T, hf)
+            return u - float(u0) - lam / float(dens) * conv * float(dt)
+
+        lo, hi = float(u1) * (1 - 1.2e-5), float(u1) * (1 + 1.2e-5)
+        bracketed += f(lo) <= 0.0 <= f(hi) or f(hi) <= 0.0 <= f(lo)
+        checked += 1
+    assert checked > 150
+    assert bracketed >= 0.8 * checked, (bracketed, checked)   # 179 of 200 here
+
+
+def test_photoionization_equilibrium_after_a_long_step():
+    """Strong HI photoionization, no heating: after many recombination
+    times x_HI (k1 ne + Gamma) = k2 x_HII ne at the final temperature."""
+    hf = ic.helium_fact(0.24)
+    p, _ = units(hf)
+    c = np.zeros(13, np.float32)
+    c[2], c[3] = 0.5, 0.5
+    c[4] = 1.0
+    c[1] = c[3]
+    c[0] = ic.wmol(c[1])
+    c[7] = 1e-12                      # GammaHI [1/s]; no heating, no helium rates
+    dens = np.float32(100.0)          # n_H = 0.1 cm^-3
+    u0 = np.float32(1e4 / (p.uenetok * c[0]))
+    co, uo, ns, _ = O.chem_advance(dens, u0, u0, c, p, np.float32(0.05))
+    T = float(uo) * p.uenetok * float(ic.wmol(co[1]))
+    k = O.chem_k(T)
+    nH = float(dens) * p.denstonh
+    ne = nH * float(co[1])
+    lhs = float(co[2]) * (k[0] * ne + 1e-12)
+    rhs = k[1] * float(co[3]) * ne
+    assert ns > 10
+    assert abs(lhs - rhs) <= 1e-3 * rhs, (lhs, rhs, T)
