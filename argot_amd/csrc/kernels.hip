@@ -97,9 +97,9 @@ __device__ __forceinline__ float sinhc_small(float y) {
   return __builtin_fmaf(y2, p, 1.0f);
 }
 
-// Bins [k0, k1) when every in-cell optical depth of the wavefront's rays is
-// at most 4: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y
-// with y = dtau/2, one exponential per bin.
+// Bins [k0, k1) of a ray whose in-cell optical depth is at most 4 in every
+// bin: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y with
+// y = dtau/2, one exponential per bin.
 template <int NS, typename SpecF>
 __device__ __forceinline__ void rate_bins_thin(const RateTables& rt, int k0, int k1, float cst,
                                                const float n[3], const float d[3], SpecF spec,
@@ -188,9 +188,12 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   const float c1 = (float)(-(rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2]));
   const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
   double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
-  // largest in-cell optical depth of this ray over the bins
+  // largest in-cell optical depth of this ray over the bins.  The choice is
+  // the lane's own, so a cell's rates do not depend on which rays share its
+  // wavefront (tile shape, grid size); lanes of both kinds in one wavefront
+  // run both loops under their masks.
   const double dmax = rr.d[0] * rt.cs_max[0] + rr.d[1] * rt.cs_max[1] + rr.d[2] * rt.cs_max[2];
-  if (__all(dmax <= 4.0)) {
+  if (dmax <= 4.0) {
     rate_bins_thin<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
     rate_bins_thin<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
     rate_bins_thin<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
