@@ -36,7 +36,9 @@ EXPORTS = [
     "argot_hip_graph_captures", "argot_hip_autotune_step", "argot_hip_last_error_detail",
     "smooth_photoion_rate", "step_chemistry_gpu", "step_chemistry", "calc_dtime_gpu",
     "calc_timestep_chem", "update_chemistry_gpu", "argot_hip_chem_stats",
-    "argot_hip_set_iterations",
+    "argot_hip_set_iterations", "input_mesh_single", "input_mesh", "input_src_file", "input_src",
+    "input_data", "output_mesh_single", "output_mesh", "output_src", "output_data",
+    "argot_hip_set_max_sources",
 ]
 
 _lib = None
@@ -132,6 +134,14 @@ def lib():
     L.update_chemistry_gpu.argtypes = [cm, cp, rp]
     L.argot_hip_chem_stats.argtypes = [P(abi.ChemStats)]
     L.argot_hip_set_iterations.argtypes = [C.c_int]
+    vp, cs_ = C.c_void_p, C.c_char_p
+    for n in ("input_mesh_single", "input_mesh", "output_mesh_single", "output_mesh"):
+        getattr(L, n).argtypes = [vp, rp, cs_]
+    for n in ("input_src_file", "input_src", "output_src"):
+        getattr(L, n).argtypes = [vp, rp, cs_]
+    L.input_data.argtypes = [vp, vp, rp, cs_]
+    L.output_data.argtypes = [vp, vp, rp, cs_]
+    L.argot_hip_set_max_sources.argtypes = [C.c_uint64]
     L.argot_hip_set_tile.argtypes = [C.c_int, C.c_int, C.c_int]
     L.argot_hip_set_graph.argtypes = [C.c_int]
     L.argot_hip_graph_captures.restype = C.c_int64

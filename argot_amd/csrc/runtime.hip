@@ -261,6 +261,7 @@ const char* argot_hip_error_string(int code) {
     case ARGOT_ERR_STATE: return "init_gpu / send_mesh_data not called, or no trained surrogate";
     case ARGOT_ERR_NOMEM: return "device allocation failed";
     case ARGOT_ERR_TOO_MANY_SEGMENTS: return "ray crosses too many domains";
+    case ARGOT_ERR_IO: return "file could not be opened or is short";
     default: return "unknown";
   }
 }

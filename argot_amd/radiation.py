@@ -108,6 +108,36 @@ def fill_mesh(w):
     return mesh
 
 
+class _StateInfo:
+    def __init__(self, n, helium_fact, name):
+        self.n, self.helium_fact, self.name = n, helium_fact, name
+
+
+def write_snapshot(prefix, mesh, src, run):
+    """output_data: '<prefix>_XXX_YYY_ZZZ' (run_param + per-cell records) and
+    '<prefix>_src.dat' (sources)."""
+    lib().output_data(mesh, src, C.byref(run), prefix.encode())
+    check("output_data")
+
+
+def read_snapshot(prefix, nmesh, max_sources=1024, rank=(0, 0, 0)):
+    """input_data into new host arrays for a local mesh of nmesh^3 cells and
+    at most max_sources sources: (mesh, src, run_param)."""
+    L = lib()
+    run = abi.RunParam()
+    run.nmesh_x_local = run.nmesh_y_local = run.nmesh_z_local = nmesh
+    run.rank_x, run.rank_y, run.rank_z = rank
+    mesh = abi.mesh_array(nmesh ** 3)
+    src = (abi.RadiationSrc * max(max_sources, 1))()
+    L.argot_hip_set_max_sources(max_sources)
+    try:
+        L.input_data(mesh, src, C.byref(run), prefix.encode())
+    finally:
+        L.argot_hip_set_max_sources(1024)
+    check("input_data")
+    return mesh, src, run
+
+
 class PointSourceStep:
     """One GPU, one rank: the reference's GPU-ARGOT call sequence."""
 
@@ -126,6 +156,30 @@ class PointSourceStep:
         check("init_gpu")
         L.send_mesh_data(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
         check("send_mesh_data")
+
+    @classmethod
+    def from_state(cls, mesh, src, run, helium_fact, tau_mode=0, rate_mode=0):
+        """A step over given host arrays (e.g. read by read_snapshot): the
+        tables are set up again (setup_freq_param / setup_cross_section, as
+        init_run does) and the mesh is sent to the device."""
+        L = lib()
+        self = cls.__new__(cls)
+        n = run.nmesh_x_local
+        self.w = _StateInfo(n=n, helium_fact=helium_fact, name="snapshot")
+        self.freq, self.cs = freq_and_csect()
+        self.run = run
+        self.run.freq = self.freq
+        C.memmove(C.addressof(self.run.csect), C.addressof(self.cs), C.sizeof(self.cs))
+        self.src = src
+        self.mesh = mesh
+        self.cmem = abi.CudaMemSpace()
+        self.cpar = abi.CudaParam()
+        self.tau_mode, self.rate_mode = tau_mode, rate_mode
+        L.argot_hip_set_helium_fact(helium_fact)
+        L.init_gpu(self.mesh, C.byref(self.cmem), C.byref(self.cpar), C.byref(self.run))
+        check("init_gpu")
+        self.send_mesh()
+        return self
 
     def step(self, dtime=0.0, iterations=0):
         """step_radiation_tree: one point-source pass (iterations=0), or the

@@ -274,6 +274,44 @@ double calc_timestep_chem(struct cuda_mem_space *cuda_mem, struct cuda_param *th
 void update_chemistry_gpu(struct cuda_mem_space *cuda_mem, struct cuda_param *this_cuda,
                           struct run_param *this_run);
 
+/* On-disk formats (host only; no GPU needed).  Mesh file
+ * "<prefix>_XXX_YYY_ZZZ" (rank coordinates): struct run_param as it lies in
+ * memory, then one struct fluid_mesh_io (dens eneg momx momy momz uene pot
+ * chem, 80 B) per local cell in mesh order.  Source file "<prefix>_src.dat":
+ * uint64 nsrc, struct freq_param, nsrc struct radiation_src.
+ * The reader checks the file's header against the caller's run_param:
+ * nmesh_*_local must equal the caller's (the mesh it allocated); the global
+ * mesh and rank counts must equal the caller's when those are set; nspecies
+ * and nchannel must be this build's (5, 3).  A mismatch gives ARGOT_ERR_ARG
+ * (the reference exits), an unreadable or short file ARGOT_ERR_IO; on
+ * error nothing is written to mesh, src or this_run. */
+/* replaces input_mesh_single, src/input_data.c:117-150: this_run := the
+ * file's header (pointer fields included, as the reference reads them);
+ * mesh[].{dens,eneg,mom*,uene,pot,chem} from the records, prev_chem := chem,
+ * prev_uene := uene; other fields untouched. */
+void input_mesh_single(struct fluid_mesh *mesh, struct run_param *this_run, char *filename);
+/* replaces input_mesh, src/input_data.c:152-162 (file "<prefix>_XXX_YYY_ZZZ") */
+void input_mesh(struct fluid_mesh *mesh, struct run_param *this_run, char *prefix);
+/* replaces input_src_file, src/input_data.c:192-217: this_run->nsrc and
+ * ->freq from the file, src[0..nsrc) from its records; nsrc above
+ * argot_hip_set_max_sources gives ARGOT_ERR_ARG (reference: NSOURCE_MAX). */
+void input_src_file(struct radiation_src *src, struct run_param *this_run, char *src_file);
+/* replaces input_src, src/input_data.c:164-190 (file "<prefix>_src.dat") */
+void input_src(struct radiation_src *src, struct run_param *this_run, char *prefix);
+/* replaces input_data, src/input_data.c:220-230 (no cosmology) */
+void input_data(struct fluid_mesh *mesh, struct radiation_src *src, struct run_param *this_run,
+                char *prefix);
+/* replaces output_mesh_single / output_mesh, src/output_data.c:14-57 */
+void output_mesh_single(struct fluid_mesh *mesh, struct run_param *this_run, char *filename);
+void output_mesh(struct fluid_mesh *mesh, struct run_param *this_run, char *prefix);
+/* replaces output_src / output_data, src/output_data.c:60-86 */
+void output_src(struct radiation_src *src, struct run_param *this_run, char *prefix);
+void output_data(struct fluid_mesh *mesh, struct radiation_src *src, struct run_param *this_run,
+                 char *prefix);
+/* Capacity of the caller's source array for input_src (default 1024,
+ * NSOURCE_MAX of src/run_param.h:74). */
+void argot_hip_set_max_sources(uint64_t n);
+
 /* Counters of the last step_chemistry(_gpu) call. */
 struct argot_chem_stats {
     uint64_t ncell;          /* cells */
@@ -398,7 +436,8 @@ enum argot_hip_status {
     ARGOT_ERR_STATE = 3,        /* init_gpu not called / mesh not sent /
                                    rate mode 1 without a trained surrogate */
     ARGOT_ERR_NOMEM = 4,        /* device allocation failed */
-    ARGOT_ERR_TOO_MANY_SEGMENTS = 5  /* ray crosses too many domains */
+    ARGOT_ERR_TOO_MANY_SEGMENTS = 5, /* ray crosses too many domains */
+    ARGOT_ERR_IO = 6            /* a file could not be opened, or is short */
 };
 
 /* Status of the last entry point called on this thread (0 = OK). */
