@@ -54,7 +54,7 @@ def blackbody_sources(pos, rates, temps, freq=None):
 def run_param_for(w, freq, cs):
     """struct run_param of one rank owning the whole box [0,1)^3."""
     r = abi.RunParam()
-    n = w.n
+    nx, ny, nz = w.dens.shape
     r.helium_flag = 1
     r.ngrid_nu = abi.NGRID_NU
     r.mpi_nproc = 1
@@ -74,19 +74,19 @@ def run_param_for(w, freq, cs):
     r.xmax = r.ymax = r.zmax = 1.0
     r.xmin_local = r.ymin_local = r.zmin_local = 0.0
     r.xmax_local = r.ymax_local = r.zmax_local = 1.0
-    r.nmesh_x_total = r.nmesh_y_total = r.nmesh_z_total = n
-    r.nmesh_x_local = r.nmesh_y_local = r.nmesh_z_local = n
-    d = np.float32(np.float32(1.0) / np.float32(n))
-    r.delta_x = r.delta_y = r.delta_z = float(d)
+    r.nmesh_x_total, r.nmesh_y_total, r.nmesh_z_total = nx, ny, nz
+    r.nmesh_x_local, r.nmesh_y_local, r.nmesh_z_local = nx, ny, nz
+    r.delta_x, r.delta_y, r.delta_z = (float(np.float32(np.float32(1.0) / np.float32(k)))
+                                       for k in (nx, ny, nz))
     r.theta_crit = 0.7
-    r.nmesh_per_loop = n * n * n
+    r.nmesh_per_loop = nx * ny * nz
     return r
 
 
 def fill_mesh(w):
     """struct fluid_mesh[n^3] holding the workload's density, fractions and
     temperature (uene = prev_uene = T / (uenetok wmol), felec from the ions)."""
-    ncell = w.n ** 3
+    ncell = w.dens.size
     mesh = abi.mesh_array(ncell)
     v = abi.mesh_view(mesh)
     v["dens"] = w.dens.reshape(-1)

@@ -52,6 +52,22 @@ def small_workload(n=16, nsrc=5, seed=7, lognormal=True, inside_cells=True):
                        np.full(nsrc, 1.0e5, np.float32))
 
 
+def ragged_workload(shape, nsrc=4, seed=17):
+    """A non-cubic mesh (sizes that are not multiples of the 4x4x4 target
+    tile, non-cubic cells) with sources off the cell centres."""
+    rng = np.random.default_rng(seed)
+    dens = np.exp(rng.normal(0.0, 0.7, shape)).astype(np.float32)
+    fHI = rng.uniform(1e-4, 1.0, shape).astype(np.float32)
+    fHeI = rng.uniform(1e-4, 1.0, shape).astype(np.float32)
+    fHII = (1 - fHI).astype(np.float32)
+    fHeII = (1 - fHeI).astype(np.float32)
+    pos = rng.uniform(0.02, 0.98, (nsrc, 3)).astype(np.float32)
+    rates = ic.power_law_rates(rng, nsrc) if nsrc else np.zeros(0)
+    return ic.Workload("ragged", shape[0], dens, fHI, fHII, fHeI, fHeII,
+                       np.zeros(shape, np.float32), pos, rates,
+                       np.full(nsrc, 1.0e5, np.float32))
+
+
 def rel_err(a, b, floor):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

@@ -6,7 +6,7 @@ import pytest
 
 from argot_amd import ic
 from argot_amd import radiation as R
-from helpers import cell_centre, oracle_problem, rel_err, small_workload
+from helpers import cell_centre, oracle_problem, ragged_workload, rel_err, small_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -104,3 +104,30 @@ def test_deterministic(require_gpu):
     step.step()
     b = step.rates()
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("shape", [(12, 7, 5), (1, 9, 3), (2, 2, 2)])
+def test_ragged_mesh_every_ray(require_gpu, shape):
+    """Mesh sizes that are not multiples of the target tile and non-cubic
+    cells: every ray, and every cell's rates."""
+    w = ragged_workload(shape, nsrc=4, seed=sum(shape))
+    P = oracle_problem(w)
+    step = R.PointSourceStep(w)
+    rec, st = step.capture(200000)
+    assert st["nray"] == len(rec) and st["ntarget"] == np.prod(shape)
+    compare_records(rec, P, shape)
+    gpu = step.rates()
+    r32, _, ost = P.step()
+    assert ost["nray"] == st["nray"] and ost["ncross"] == st["ncross"]
+    fin = np.isfinite(r32)
+    assert np.array_equal(fin, np.isfinite(gpu))
+    err = rel_err(gpu[fin], r32[fin], RATE_FLOOR)
+    assert err.max() < RATE_TOL, f"max rel err {err.max():.3e}"
+
+
+def test_no_sources_gives_zero_rates(require_gpu):
+    w = ragged_workload((8, 8, 8), nsrc=0, seed=2)
+    step = R.PointSourceStep(w)
+    st = step.step()
+    assert st["nray"] == 0 and st["ncross"] == 0
+    assert np.all(step.rates() == 0.0)
