@@ -108,6 +108,7 @@ struct ChemCounters {
   unsigned int pad;
   unsigned long long nsub;
   unsigned long long niter;
+  unsigned long long next;   // cell queue of the persistent chemistry kernel
 };
 
 }  // namespace argot

@@ -306,100 +306,115 @@ __device__ __forceinline__ float implicit_update(float n, float s, float a, doub
   return (float)(((double)s * dt + (double)n) / (1.0 + (double)a * dt));
 }
 
-struct StepOut {
-  int nsub;      // sub-steps taken
-  int niter;     // bisection iterations, summed over sub-steps
-};
 
 // The reaction network and the heating/cooling integrated together over
 // dtime (code units), advance_reaction_and_heatcool_dev
-// (step_reaction.cu:402-733).  c enters as prev_chem (with the cell's
-// photoionization and heating rates) and leaves as the updated state; *uene
-// enters as prev_uene.  mesh_uene is the mesh's current specific energy,
-// which the reference reads for the first temperature and for every
-// safe-step estimate.
-__device__ inline StepOut advance_reaction_and_heatcool(float dens, float mesh_uene, float* uene,
-                                                        State* cp, const ChemParams& p,
-                                                        float dtime) {
-  State& c = *cp;
-  StepOut out{0, 0};
-  const double dtime_p = (double)dtime * p.tunit;
-  Dens n = number_densities(dens, c, p);
-  const float nH = n.nH;
-  float nHI = n.nHI, nHII = n.nHII, nHeI = n.nHeI, nHeII = n.nHeII, nHeIII = n.nHeIII;
-  float ne = n.ne;
-  const float nHe = n.nHe;
-  const float GHI = c.GammaHI;
-  const double GHeI = (double)c.GammaHeI, GHeII = (double)c.GammaHeII;
+// (step_reaction.cu:402-733), split into its set-up (cell_begin) and its
+// sub-steps (cell_substep, while cell_more) so that a kernel can interleave
+// cells.  c enters as prev_chem (with the cell's photoionization and
+// heating rates) and leaves as the updated state; uene enters as prev_uene.
+// mesh_uene is the mesh's current specific energy, which the reference reads
+// for the first temperature and for every safe-step estimate.
+struct CellRun {
+  State c;
+  float dens, mesh_uene, uene;
+  float nH, nHe, nHI, nHII, nHeI, nHeII, nHeIII, ne;
+  float GHI;
+  double GHeI, GHeII;
+  KSet K;
+  double dtime_p, dtime_safe, elapsed;
+  int nsub, niter;
+};
 
-  float T = temperature(mesh_uene, c, p);
-  KSet K = rate_coefficients(T);
+__device__ inline void cell_begin(CellRun& r, const ChemParams& p, float dtime) {
+  r.nsub = r.niter = 0;
+  r.dtime_p = (double)dtime * p.tunit;
+  const Dens n = number_densities(r.dens, r.c, p);
+  r.nH = n.nH;
+  r.nHe = n.nHe;
+  r.nHI = n.nHI;
+  r.nHII = n.nHII;
+  r.nHeI = n.nHeI;
+  r.nHeII = n.nHeII;
+  r.nHeIII = n.nHeIII;
+  r.ne = n.ne;
+  r.GHI = r.c.GammaHI;
+  r.GHeI = (double)r.c.GammaHeI;
+  r.GHeII = (double)r.c.GammaHeII;
+  r.K = rate_coefficients(temperature(r.mesh_uene, r.c, p));
   // calc_dtime reads the same cell state and temperature here: same K.
-  double dtime_safe = kDtFactor * (double)safe_dtime(n, c, K);
-  if (dtime_p < dtime_safe) dtime_safe = dtime_p;
+  r.dtime_safe = kDtFactor * (double)safe_dtime(n, r.c, r.K);
+  if (r.dtime_p < r.dtime_safe) r.dtime_safe = r.dtime_p;
+  r.elapsed = 0.0;
+}
 
-  double elapsed = 0.0;
-  while (elapsed < dtime_p) {
-    if (elapsed + dtime_safe > dtime_p) dtime_safe = dtime_p - elapsed;
-    elapsed += dtime_safe;
-    out.nsub++;
+__device__ __forceinline__ bool cell_more(const CellRun& r) { return r.elapsed < r.dtime_p; }
 
-    nHI = implicit_update(nHI, K.k2 * nHII * ne, K.k1 * ne + GHI, dtime_safe);
-    nHII = implicit_update(nHII, K.k1 * nHI * ne + GHI * nHI, K.k2 * ne, dtime_safe);
-    nHeI = implicit_update(nHeI, K.k4 * nHeII * ne, (float)((double)(K.k3 * ne) + GHeI),
-                           dtime_safe);
-    nHeII = implicit_update(
-        nHeII, (float)((double)((K.k3 * nHeI + K.k6 * nHeIII) * ne) + GHeI * (double)nHeI),
-        (float)((double)((K.k4 + K.k5) * ne) + GHeII), dtime_safe);
-    nHeIII = implicit_update(nHeIII, (float)((double)(K.k5 * nHeII * ne) + GHeII * (double)nHeII),
-                             K.k6 * ne, dtime_safe);
-    {
-      float s = K.k1 * nHI * ne + GHI * nHI;
-      float a = K.k2 * nHII;
-      s = (float)((double)s + ((double)(K.k3 * nHeI * ne + K.k5 * nHeII * ne) +
-                               GHeI * (double)nHeI + GHeII * (double)nHeII));
-      a = a + (K.k4 * nHeII + K.k6 * nHeIII);
-      ne = implicit_update(ne, s, a, dtime_safe);
-    }
+__device__ inline void cell_substep(CellRun& r, const ChemParams& p) {
+  State& c = r.c;
+  const KSet& K = r.K;
+  if (r.elapsed + r.dtime_safe > r.dtime_p) r.dtime_safe = r.dtime_p - r.elapsed;
+  r.elapsed += r.dtime_safe;
+  r.nsub++;
+  const double dts = r.dtime_safe;
+  float nHI = r.nHI, nHII = r.nHII, nHeI = r.nHeI, nHeII = r.nHeII, nHeIII = r.nHeIII;
+  float ne = r.ne;
+  const float GHI = r.GHI;
+  const double GHeI = r.GHeI, GHeII = r.GHeII;
 
-    // fractions, renormalised per element; electrons from the ions
-    c.fHI = nHI / nH;
-    c.fHII = nHII / nH;
-    float cf = 1.0f / (c.fHI + c.fHII);
-    c.fHI *= cf;
-    c.fHII *= cf;
-    const double nHe_t = (double)nHe + kTiny;
-    c.fHeI = (float)((double)nHeI / nHe_t);
-    c.fHeII = (float)((double)nHeII / nHe_t);
-    c.fHeIII = (float)((double)nHeIII / nHe_t);
-    cf = (float)(1.0 / ((double)(c.fHeI + c.fHeII + c.fHeIII) + kTiny));
-    c.fHeI *= cf;
-    c.fHeII *= cf;
-    c.fHeIII *= cf;
-    c.felec = c.fHII;
-    c.felec = (float)((double)c.felec +
-                      (p.helium_fact * ((double)c.fHeII + 2.0 * (double)c.fHeIII) + kTiny));
-
-    dtime_safe /= p.tunit;
-    out.niter += advance_heatcool(dens, uene, c, p, (float)dtime_safe);
-
-    // densities, temperature and coefficients for the next sub-step
-    n = number_densities(dens, c, p);
-    nHI = n.nHI;
-    nHII = n.nHII;
-    nHeI = n.nHeI;
-    nHeII = n.nHeII;
-    nHeIII = n.nHeIII;
-    ne = n.ne;
-    T = temperature(*uene, c, p);
-    K = rate_coefficients(T);
-    // calc_dtime takes the temperature of the mesh's (unchanged) energy;
-    // its coefficients equal K only when the two temperatures agree.
-    const float Tm = temperature(mesh_uene, c, p);
-    const KSet Km = Tm == T ? K : rate_coefficients(Tm);
-    dtime_safe = kDtFactor * (double)safe_dtime(n, c, Km);
+  nHI = implicit_update(nHI, K.k2 * nHII * ne, K.k1 * ne + GHI, dts);
+  nHII = implicit_update(nHII, K.k1 * nHI * ne + GHI * nHI, K.k2 * ne, dts);
+  nHeI = implicit_update(nHeI, K.k4 * nHeII * ne, (float)((double)(K.k3 * ne) + GHeI), dts);
+  nHeII = implicit_update(
+      nHeII, (float)((double)((K.k3 * nHeI + K.k6 * nHeIII) * ne) + GHeI * (double)nHeI),
+      (float)((double)((K.k4 + K.k5) * ne) + GHeII), dts);
+  nHeIII = implicit_update(nHeIII, (float)((double)(K.k5 * nHeII * ne) + GHeII * (double)nHeII),
+                           K.k6 * ne, dts);
+  {
+    float s = K.k1 * nHI * ne + GHI * nHI;
+    float a = K.k2 * nHII;
+    s = (float)((double)s + ((double)(K.k3 * nHeI * ne + K.k5 * nHeII * ne) +
+                             GHeI * (double)nHeI + GHeII * (double)nHeII));
+    a = a + (K.k4 * nHeII + K.k6 * nHeIII);
+    ne = implicit_update(ne, s, a, dts);
   }
-  return out;
+
+  // fractions, renormalised per element; electrons from the ions
+  c.fHI = nHI / r.nH;
+  c.fHII = nHII / r.nH;
+  float cf = 1.0f / (c.fHI + c.fHII);
+  c.fHI *= cf;
+  c.fHII *= cf;
+  const double nHe_t = (double)r.nHe + kTiny;
+  c.fHeI = (float)((double)nHeI / nHe_t);
+  c.fHeII = (float)((double)nHeII / nHe_t);
+  c.fHeIII = (float)((double)nHeIII / nHe_t);
+  cf = (float)(1.0 / ((double)(c.fHeI + c.fHeII + c.fHeIII) + kTiny));
+  c.fHeI *= cf;
+  c.fHeII *= cf;
+  c.fHeIII *= cf;
+  c.felec = c.fHII;
+  c.felec = (float)((double)c.felec +
+                    (p.helium_fact * ((double)c.fHeII + 2.0 * (double)c.fHeIII) + kTiny));
+
+  r.dtime_safe = dts / p.tunit;
+  r.niter += advance_heatcool(r.dens, &r.uene, c, p, (float)r.dtime_safe);
+
+  // densities, temperature and coefficients for the next sub-step
+  const Dens n = number_densities(r.dens, c, p);
+  r.nHI = n.nHI;
+  r.nHII = n.nHII;
+  r.nHeI = n.nHeI;
+  r.nHeII = n.nHeII;
+  r.nHeIII = n.nHeIII;
+  r.ne = n.ne;
+  const float T = temperature(r.uene, c, p);
+  r.K = rate_coefficients(T);
+  // calc_dtime takes the temperature of the mesh's (unchanged) energy;
+  // its coefficients equal K only when the two temperatures agree.
+  const float Tm = temperature(r.mesh_uene, c, p);
+  const KSet Km = Tm == T ? r.K : rate_coefficients(Tm);
+  r.dtime_safe = kDtFactor * (double)safe_dtime(n, c, Km);
 }
 
 // Relative change of the HI photoionization rate and of the electron

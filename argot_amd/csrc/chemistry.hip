@@ -2,7 +2,8 @@
 //
 //   chem_step     advance_reaction_and_heatcool_kernel (chemistry.cu:390-433):
 //                 prev_chem/prev_uene -> chem/uene over dtime, maxima of the
-//                 relative changes
+//                 relative changes; one cell per thread, or persistent with a
+//                 cell queue (chem_step_queue)
 //   chem_dtime    calc_dtime_gpu_kernel (chemistry.cu:246-269): minimum safe
 //                 step over cells
 //   chem_update   update_chemistry_kernel (chemistry.cu:224-234): prev := new,
@@ -12,6 +13,8 @@
 // double-precision arithmetic (transcendentals in the rate fits and the
 // bisection), so each thread reads its record's fields directly.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "argot_internal.h"
 #include "chem_device.h"
@@ -69,6 +72,7 @@ __device__ __forceinline__ void wave_sum_to(unsigned long long v, unsigned long 
   if ((threadIdx.x & 63) == 0) atomicAdd(dst, v);
 }
 
+// One cell per thread, all of its sub-steps in turn.
 __global__ void __launch_bounds__(kBlock) chem_step_kernel(unsigned char* __restrict__ mesh,
                                                            int64_t ncell, ChemParams p,
                                                            float dtime,
@@ -78,19 +82,85 @@ __global__ void __launch_bounds__(kBlock) chem_step_kernel(unsigned char* __rest
   unsigned long long nsub = 0, niter = 0;
   if (c < ncell) {
     unsigned char* m = mesh + c * kStride;
-    const float dens = ld(m, kDens);
-    const float mesh_uene = ld(m, kUene);
-    float uene = ld(m, kPrevUene);
-    chem::State s = ld_state(m, kPrevChem);
+    chem::CellRun r;
+    r.dens = ld(m, kDens);
+    r.mesh_uene = ld(m, kUene);
+    r.uene = ld(m, kPrevUene);
+    r.c = ld_state(m, kPrevChem);
     st(m, kDuene, 0.0f);
-    const chem::StepOut o = chem::advance_reaction_and_heatcool(dens, mesh_uene, &uene, &s, p, dtime);
+    chem::cell_begin(r, p, dtime);
+    while (chem::cell_more(r)) chem::cell_substep(r, p);
     const chem::State old = ld_state(m, kChem);
-    dchem = __float_as_uint(chem::chem_change(s, old));
-    duene = __float_as_uint(chem::uene_change(uene, mesh_uene));
-    st(m, kUene, uene);
-    st_state(m, kChem, s);
-    nsub = (unsigned long long)o.nsub;
-    niter = (unsigned long long)o.niter;
+    dchem = __float_as_uint(chem::chem_change(r.c, old));
+    duene = __float_as_uint(chem::uene_change(r.uene, r.mesh_uene));
+    st(m, kUene, r.uene);
+    st_state(m, kChem, r.c);
+    nsub = (unsigned long long)r.nsub;
+    niter = (unsigned long long)r.niter;
+  }
+  wave_reduce_to<true>(dchem, &ctr->diff_chem);
+  wave_reduce_to<true>(duene, &ctr->diff_uene);
+  wave_sum_to(nsub, &ctr->nsub);
+  wave_sum_to(niter, &ctr->niter);
+}
+
+// Persistent form: a lane whose cell is done takes the next one from a
+// queue (one atomic per wavefront refill), so a wavefront stays full while
+// a few of its cells need many sub-steps.  Each cell's arithmetic is the
+// same as in chem_step_kernel; only the interleaving of cells differs.
+__global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* __restrict__ mesh,
+                                                                 int64_t ncell, ChemParams p,
+                                                                 float dtime,
+                                                                 ChemCounters* __restrict__ ctr) {
+  const int lane = threadIdx.x & 63;
+  unsigned int dchem = 0u, duene = 0u;
+  unsigned long long nsub = 0, niter = 0;
+  chem::CellRun r;
+  int64_t cell = -1;       // the lane's current cell, -1: needs one
+  bool drained = false;    // the queue had no cell left for this lane
+  for (;;) {
+    const bool need = cell < 0 && !drained;
+    const unsigned long long want = __ballot(need);
+    if (want) {
+      const int leader = __builtin_ctzll(want);
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(&ctr->next, (unsigned long long)__popcll(want));
+      base = __shfl(base, leader, 64);
+      if (need) {
+        const unsigned long long idx =
+            base + (unsigned long long)__popcll(want & ((1ull << lane) - 1ull));
+        if (idx < (unsigned long long)ncell) {
+          cell = (int64_t)idx;
+          unsigned char* m = mesh + cell * kStride;
+          r.dens = ld(m, kDens);
+          r.mesh_uene = ld(m, kUene);
+          r.uene = ld(m, kPrevUene);
+          r.c = ld_state(m, kPrevChem);
+          st(m, kDuene, 0.0f);
+          chem::cell_begin(r, p, dtime);
+        } else {
+          drained = true;
+        }
+      }
+    }
+    if (__all(cell < 0)) break;   // every lane drained and idle
+    if (cell >= 0) {
+      if (chem::cell_more(r)) {
+        chem::cell_substep(r, p);
+      } else {
+        unsigned char* m = mesh + cell * kStride;
+        const chem::State old = ld_state(m, kChem);
+        const unsigned int a = __float_as_uint(chem::chem_change(r.c, old));
+        const unsigned int b = __float_as_uint(chem::uene_change(r.uene, r.mesh_uene));
+        dchem = a > dchem ? a : dchem;
+        duene = b > duene ? b : duene;
+        st(m, kUene, r.uene);
+        st_state(m, kChem, r.c);
+        nsub += (unsigned long long)r.nsub;
+        niter += (unsigned long long)r.niter;
+        cell = -1;
+      }
+    }
   }
   wave_reduce_to<true>(dchem, &ctr->diff_chem);
   wave_reduce_to<true>(duene, &ctr->diff_uene);
@@ -133,6 +203,7 @@ __global__ void chem_reset_kernel(ChemCounters* ctr) {
     ctr->pad = 0u;
     ctr->nsub = 0ull;
     ctr->niter = 0ull;
+    ctr->next = 0ull;
   }
 }
 
@@ -141,10 +212,15 @@ unsigned int nblocks(int64_t ncell) { return (unsigned int)((ncell + kBlock - 1)
 }  // namespace
 
 hipError_t launch_chem_step(void* mesh, int64_t ncell, const ChemParams& p, float dtime,
-                            ChemCounters* ctr, hipStream_t s) {
+                            ChemCounters* ctr, int queue_blocks, hipStream_t s) {
   chem_reset_kernel<<<1, 64, 0, s>>>(ctr);
-  if (ncell > 0)
+  if (ncell <= 0) return hipGetLastError();
+  if (queue_blocks > 0) {
+    const unsigned int nb = (unsigned int)std::min<int64_t>(queue_blocks, nblocks(ncell));
+    chem_step_queue_kernel<<<nb, kBlock, 0, s>>>((unsigned char*)mesh, ncell, p, dtime, ctr);
+  } else {
     chem_step_kernel<<<nblocks(ncell), kBlock, 0, s>>>((unsigned char*)mesh, ncell, p, dtime, ctr);
+  }
   return hipGetLastError();
 }
 

@@ -135,3 +135,28 @@ def test_iteration_loop_equals_its_calls(require_gpu):
     # smooth() also writes the smoothed rates into the host mesh, the loop
     # does not: compare what lives on the device
     assert np.array_equal(got, want)
+
+
+def test_queue_kernel_equals_one_cell_per_thread(require_gpu):
+    """The persistent cell-queue kernel and the one-cell-per-thread kernel
+    give identical meshes and counts."""
+    from argot_amd._lib import lib
+    L = lib()
+    st = stepped(n=16, seed=44)
+    ref = np.frombuffer(st.mesh, np.uint8).copy()
+    try:
+        L.argot_hip_set_chem_queue(0)
+        d0 = st.chemistry(np.float32(3e-3))
+        c0 = st.chem_stats()
+        m0 = st.mesh_bytes()
+        np.copyto(np.frombuffer(st.mesh, np.uint8), ref)
+        st.send_mesh()
+        L.argot_hip_set_chem_queue(1)
+        d1 = st.chemistry(np.float32(3e-3))
+        c1 = st.chem_stats()
+        m1 = st.mesh_bytes()
+    finally:
+        L.argot_hip_set_chem_queue(1)
+    assert np.array_equal(m0, m1)
+    assert d0 == d1
+    assert (c0["nsub"], c0["niter"]) == (c1["nsub"], c1["niter"])
