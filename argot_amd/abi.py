@@ -120,7 +120,8 @@ class StepStats(C.Structure):
     _fields_ = [("ntarget", C.c_uint64), ("nray", C.c_uint64), ("ncross", C.c_uint64),
                 ("nfallback", C.c_uint64), ("ms_build", C.c_double), ("ms_kernel", C.c_double),
                 ("ms_total", C.c_double), ("nsurrogate", C.c_uint64),
-                ("iterations", C.c_uint64), ("ms_chem", C.c_double)]
+                ("iterations", C.c_uint64), ("ms_chem", C.c_double),
+                ("nvisit", C.c_uint64), ("nemit_iter", C.c_uint64)]
 
 
 class ChemStats(C.Structure):

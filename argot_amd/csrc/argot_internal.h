@@ -88,6 +88,8 @@ struct StepCounters {
   unsigned long long nfallback;
   unsigned long long nrecord;
   unsigned long long nsurrogate;
+  unsigned long long nvisit;       // tree nodes visited, summed over wavefronts
+  unsigned long long nemit_iter;   // of those, visits where some lane emitted a ray
   int queue[kNumQueues];
 };
 

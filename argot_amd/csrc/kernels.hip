@@ -388,6 +388,7 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   const int64_t ntile = (int64_t)ntx * nty * ntz;
   int q = blockIdx.x % kNumQueues, tried = 0;
   unsigned long long my_ray = 0, my_cross = 0, my_fb = 0, my_surr = 0;
+  unsigned long long my_visit = 0, my_emit_iter = 0;   // wave-uniform
   float* lds = nullptr;
   if constexpr (SURR) {
     __shared__ float surr_lds[kMlpLds];
@@ -431,6 +432,8 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
       const bool leaf = ln.n <= 1;
       const bool emit = here && (accept || ln.n == 1);
       if (here) p = (accept || leaf || ln.first < 0) ? ln.skip : ln.first;
+      my_visit++;
+      my_emit_iter += __ballot(emit) != 0 ? 1ull : 0ull;
       Vec3f src{0.0f, 0.0f, 0.0f}, st{0.0f, 0.0f, 0.0f};
       WalkOut w{};
       RayRate rr{};
@@ -512,6 +515,8 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     atomicAdd(&a.ctr->ncross, sc);
     atomicAdd(&a.ctr->nfallback, sf);
     if (SURR) atomicAdd(&a.ctr->nsurrogate, ss);
+    atomicAdd(&a.ctr->nvisit, my_visit);
+    atomicAdd(&a.ctr->nemit_iter, my_emit_iter);
   }
 }
 
@@ -607,6 +612,8 @@ __global__ void reset_counters_kernel(StepCounters* ctr) {
     ctr->nfallback = 0;
     ctr->nrecord = 0;
     ctr->nsurrogate = 0;
+    ctr->nvisit = 0;
+    ctr->nemit_iter = 0;
   }
 }
 

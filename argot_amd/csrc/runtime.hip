@@ -665,6 +665,8 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   g.stats.ncross = hc.ncross;
   g.stats.nfallback = hc.nfallback;
   g.stats.nsurrogate = hc.nsurrogate;
+  g.stats.nvisit = hc.nvisit;
+  g.stats.nemit_iter = hc.nemit_iter;
   g.stats.ms_build = t_built - t_start;
   g.stats.ms_kernel = ms;
   g.stats.ms_total = now_ms() - t_start;
@@ -733,6 +735,7 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   // src/step_radiation_tree.c:666-723: smooth, chemistry, and another pass
   // while the chemistry or the energy still moved by more than 1e-3.
   uint64_t nray = g.stats.nray, ncross = g.stats.ncross;
+  uint64_t nvisit = g.stats.nvisit, nemit = g.stats.nemit_iter;
   double ms_kernel = g.stats.ms_kernel, ms_chem = 0.0;
   int iter = 1;
   for (;;) {
@@ -754,6 +757,8 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
     if (t_err) return;
     nray += g.stats.nray;
     ncross += g.stats.ncross;
+    nvisit += g.stats.nvisit;
+    nemit += g.stats.nemit_iter;
     ms_kernel += g.stats.ms_kernel;
     iter++;
   }
@@ -762,6 +767,8 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   g.stats.iterations = (uint64_t)iter;
   g.stats.nray = nray;
   g.stats.ncross = ncross;
+  g.stats.nvisit = nvisit;
+  g.stats.nemit_iter = nemit;
   g.stats.ms_kernel = ms_kernel;
   g.stats.ms_chem = ms_chem;
   g.stats.ms_total = now_ms() - t_start;

@@ -213,8 +213,10 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = 1000.0 * wall / args.steps
     nray, ncross, ncell = st["nray"], st["ncross"], st["ntarget"]
+    util = nray / max(64 * st["nemit_iter"], 1)
     log(f"timed: {args.steps} steps, {ms_step:.1f} ms/step, kernel {np.mean(ker):.1f} ms, "
-        f"{ncross / max(nray, 1):.1f} crossings/segment")
+        f"{ncross / max(nray, 1):.1f} crossings/segment, {st['nvisit']} tree visits, "
+        f"lane utilisation {util:.3f}")
 
     kernel_ms = float(np.mean(ker))
     bytes_launch = algorithmic_bytes(nray, ncross, ncell)
@@ -249,6 +251,8 @@ def main():
                                "256 nu bins, theta 0.7, one rank",
                    "segments_per_step": int(nray), "crossings_per_step": int(ncross),
                    "tau_sum": "kahan" if args.kahan else "plain", "hip_graph": True,
+                   "tree_visits": int(st["nvisit"]), "emitting_visits": int(st["nemit_iter"]),
+                   "lane_utilisation": util,
                    "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,

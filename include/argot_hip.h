@@ -475,6 +475,10 @@ struct argot_step_stats {
     uint64_t nsurrogate;     /* rays whose rates came from the MLP surrogate */
     uint64_t iterations;     /* transfer passes (1 unless argot_hip_set_iterations) */
     double ms_chem;          /* smoothing + chemistry kernels over all passes */
+    uint64_t nvisit;         /* tree nodes visited by the wavefronts (64 targets each) */
+    uint64_t nemit_iter;     /* visits in which at least one lane emitted a ray:
+                                nray / (64 nemit_iter) is the lane utilisation of
+                                the walk and the rate loop */
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
 
