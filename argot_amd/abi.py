@@ -126,7 +126,7 @@ class StepStats(C.Structure):
 
 class ChemStats(C.Structure):
     _fields_ = [("ncell", C.c_uint64), ("nsub", C.c_uint64), ("niter", C.c_uint64),
-                ("ms_kernel", C.c_double)]
+                ("ms_kernel", C.c_double), ("nstopped", C.c_uint64)]
 
 
 class SurrogateConfig(C.Structure):

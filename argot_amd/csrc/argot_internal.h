@@ -111,6 +111,7 @@ struct ChemCounters {
   unsigned long long nsub;
   unsigned long long niter;
   unsigned long long next;   // cell queue of the persistent chemistry kernel
+  unsigned long long nstopped;   // cells ended by the NaN or sub-step guard
 };
 
 }  // namespace argot

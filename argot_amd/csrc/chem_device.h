@@ -348,7 +348,26 @@ __device__ inline void cell_begin(CellRun& r, const ChemParams& p, float dtime) 
   r.elapsed = 0.0;
 }
 
-__device__ __forceinline__ bool cell_more(const CellRun& r) { return r.elapsed < r.dtime_p; }
+// The reference sub-steps until the elapsed time reaches dtime.  Two
+// guards stop a cell earlier, where the reference would run for an
+// unbounded time: a state that has turned NaN (e.g. an infinite rate in a
+// cell that holds a source) makes every later safe step the 1-second floor,
+// and a cell is given at most kMaxSubsteps sub-steps.  Both are counted
+// (cell_stopped).
+constexpr int kMaxSubsteps = 1000000;
+
+__device__ __forceinline__ bool cell_nan(const CellRun& r) {
+  return !(r.c.felec == r.c.felec) || !(r.uene == r.uene);
+}
+
+__device__ __forceinline__ bool cell_more(const CellRun& r) {
+  return r.elapsed < r.dtime_p && r.nsub < kMaxSubsteps && !cell_nan(r);
+}
+
+// Did a guard, not the elapsed time, end this cell?
+__device__ __forceinline__ bool cell_stopped(const CellRun& r) {
+  return r.elapsed < r.dtime_p;
+}
 
 __device__ inline void cell_substep(CellRun& r, const ChemParams& p) {
   State& c = r.c;

@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock) chem_step_kernel(unsigned char* __rest
                                                            ChemCounters* __restrict__ ctr) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   unsigned int dchem = 0u, duene = 0u;
-  unsigned long long nsub = 0, niter = 0;
+  unsigned long long nsub = 0, niter = 0, nstop = 0;
   if (c < ncell) {
     unsigned char* m = mesh + c * kStride;
     chem::CellRun r;
@@ -97,11 +97,13 @@ __global__ void __launch_bounds__(kBlock) chem_step_kernel(unsigned char* __rest
     st_state(m, kChem, r.c);
     nsub = (unsigned long long)r.nsub;
     niter = (unsigned long long)r.niter;
+    nstop = chem::cell_stopped(r) ? 1ull : 0ull;
   }
   wave_reduce_to<true>(dchem, &ctr->diff_chem);
   wave_reduce_to<true>(duene, &ctr->diff_uene);
   wave_sum_to(nsub, &ctr->nsub);
   wave_sum_to(niter, &ctr->niter);
+  wave_sum_to(nstop, &ctr->nstopped);
 }
 
 // Persistent form: a lane whose cell is done takes the next one from a
@@ -114,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* 
                                                                  ChemCounters* __restrict__ ctr) {
   const int lane = threadIdx.x & 63;
   unsigned int dchem = 0u, duene = 0u;
-  unsigned long long nsub = 0, niter = 0;
+  unsigned long long nsub = 0, niter = 0, nstop = 0;
   chem::CellRun r;
   int64_t cell = -1;       // the lane's current cell, -1: needs one
   bool drained = false;    // the queue had no cell left for this lane
@@ -158,6 +160,7 @@ __global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* 
         st_state(m, kChem, r.c);
         nsub += (unsigned long long)r.nsub;
         niter += (unsigned long long)r.niter;
+        nstop += chem::cell_stopped(r) ? 1ull : 0ull;
         cell = -1;
       }
     }
@@ -166,6 +169,7 @@ __global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* 
   wave_reduce_to<true>(duene, &ctr->diff_uene);
   wave_sum_to(nsub, &ctr->nsub);
   wave_sum_to(niter, &ctr->niter);
+  wave_sum_to(nstop, &ctr->nstopped);
 }
 
 __global__ void __launch_bounds__(kBlock) chem_dtime_kernel(const unsigned char* __restrict__ mesh,
@@ -204,6 +208,7 @@ __global__ void chem_reset_kernel(ChemCounters* ctr) {
     ctr->nsub = 0ull;
     ctr->niter = 0ull;
     ctr->next = 0ull;
+    ctr->nstopped = 0ull;
   }
 }
 

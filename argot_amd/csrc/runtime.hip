@@ -716,6 +716,7 @@ static void chem_step(const run_param* r, float dtime, float* diff_chem, float* 
   g.chem_stats.ncell = (uint64_t)g.ncell;
   g.chem_stats.nsub = hc.nsub;
   g.chem_stats.niter = hc.niter;
+  g.chem_stats.nstopped = hc.nstopped;
   g.chem_stats.ms_kernel = ms;
 }
 

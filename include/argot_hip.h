@@ -318,6 +318,10 @@ struct argot_chem_stats {
     uint64_t nsub;           /* network sub-steps, summed over cells */
     uint64_t niter;          /* energy bisections, summed over cells */
     double ms_kernel;        /* chemistry kernel, HIP events */
+    uint64_t nstopped;       /* cells stopped before dtime: their state turned
+                                NaN (e.g. an infinite rate in a cell holding a
+                                source), or they reached 1e6 sub-steps; the
+                                reference would keep taking 0.1-s steps */
 };
 void argot_hip_chem_stats(struct argot_chem_stats *out);
 

@@ -246,7 +246,8 @@ void orc_chem_advance(float dens, float mesh_uene, float *uene, orc_chem *c,
     double el = 0.0;
     int ns = 0, ni = 0;
     float HI = n.HI, HII = n.HII, HeI = n.HeI, HeII = n.HeII, HeIII = n.HeIII, e = n.e;
-    while (el < tp) {
+    /* the GPU's guards: stop on a NaN state or after 1e6 sub-steps */
+    while (el < tp && ns < 1000000 && c->felec == c->felec && *uene == *uene) {
         if (el + dts > tp) dts = tp - el;
         el += dts;
         ns++;
