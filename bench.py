@@ -109,9 +109,23 @@ def surrogate_leg(step, plain_seg_s, nray):
 
 
 def chem_leg(step, dtime=1.0e-3):
-    """Chemistry of the whole mesh over dtime (code units) after the pass,
-    one thread per cell (step_chemistry); run twice, the second timed."""
-    from argot_amd import ic
+    """Chemistry of the whole mesh over dtime (code units) after the pass, as
+    the reference's loop runs it: smoothed rates, then step_chemistry; run
+    twice, the second timed.  The cells that hold a source have infinite
+    rates in this workload (sources at cell centres); they are set to zero
+    first, since such a cell's chemistry would stop on its NaN state."""
+    from argot_amd import abi, ic
+    from argot_amd import radiation as R
+    step.mesh_bytes()
+    v = abi.mesh_view(step.mesh)
+    pc = v["prev_chem"]
+    for f in R.RATE_FIELDS:
+        x = pc[f]
+        x[~np.isfinite(x)] = 0.0
+        pc[f] = x
+    v["prev_chem"] = pc
+    step.send_mesh()
+    step.smooth()
     step.chemistry(np.float32(dtime))
     t0 = time.perf_counter()
     dc, du = step.chemistry(np.float32(dtime))
@@ -124,6 +138,8 @@ def chem_leg(step, dtime=1.0e-3):
            "cells_per_s": ncell / (cs["ms_kernel"] * 1e-3),
            "substeps": int(cs["nsub"]), "bisections": int(cs["niter"]),
            "substeps_per_s": cs["nsub"] / (cs["ms_kernel"] * 1e-3),
+           "heatcool_evals_per_s": cs["niter"] / (cs["ms_kernel"] * 1e-3),
+           "cells_stopped": int(cs["nstopped"]),
            "max_diff_chem": float(dc), "max_diff_uene": float(du)}
     log(f"chemistry: {out}")
     return out
