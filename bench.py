@@ -182,6 +182,8 @@ def main():
                     help="skip the surrogate, chemistry and ART legs")
     ap.add_argument("--art-only", action="store_true",
                     help="run only the ART leg and print its JSON (profiling)")
+    ap.add_argument("--chem-only", action="store_true",
+                    help="run one point-source pass and the chemistry leg, print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -199,6 +201,10 @@ def main():
     w = ic.WORKLOADS[args.workload]()
     step = R.PointSourceStep(w, tau_mode=1 if args.kahan else 0)
     log(f"setup {time.perf_counter() - t0:.1f} s: {args.workload} n={w.n} nsrc={w.nsrc}")
+    if args.chem_only:
+        step.step()
+        print(json.dumps(chem_leg(step)), flush=True)
+        return
 
     for i in range(args.warmup):
         st = step.step()
