@@ -187,6 +187,12 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   const float d[3] = {(float)rr.d[0], (float)rr.d[1], (float)rr.d[2]};
   const float c1 = (float)(-(rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2]));
   const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
+  // per-ray prefactors first (same operations as after the loop), so that
+  // only they, not the ray's FP64 quantities, stay live through the bins
+  const double pre = scale * exp(-tsh);
+  const double p0 = rr.r[0] * rr.d[0] * pre;
+  const double p1 = rr.r[1] * rr.d[1] * pre;
+  const double p2 = rr.r[2] * rr.d[2] * pre;
   double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
   // largest in-cell optical depth of this ray over the bins.  The choice is
   // the lane's own, so a cell's rates do not depend on which rays share its
@@ -202,10 +208,6 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
     rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
     rate_bins<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
   }
-  const double pre = scale * exp(-tsh);
-  const double p0 = rr.r[0] * rr.d[0] * pre;
-  const double p1 = rr.r[1] * rr.d[1] * pre;
-  const double p2 = rr.r[2] * rr.d[2] * pre;
   out[0] = p0 * G[0];
   out[1] = p0 * H[0] * kHnuL;
   out[2] = p1 * G[1];
@@ -387,8 +389,15 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, ntz = (g.nz + TZ - 1) / TZ;
   const int64_t ntile = (int64_t)ntx * nty * ntz;
   int q = blockIdx.x % kNumQueues, tried = 0;
-  unsigned long long my_ray = 0, my_cross = 0, my_fb = 0, my_surr = 0;
-  unsigned long long my_visit = 0, my_emit_iter = 0;   // wave-uniform
+  // per-lane counts (a lane's rays and crossings over the launch fit in 32
+  // bits); wave-uniform visit counts
+  unsigned int my_ray = 0, my_cross = 0, my_fb = 0, my_surr = 0;
+  unsigned long long my_visit = 0, my_emit_iter = 0;
+  // The lane's per-cell FP64 sums and its target's density and fractions
+  // live in LDS between rays, so that the walk and the frequency loop have
+  // the registers to themselves.
+  __shared__ double acc_lds[6][64];
+  __shared__ float4 cell_lds[64];
   float* lds = nullptr;
   if constexpr (SURR) {
     __shared__ float surr_lds[kMlpLds];
@@ -408,14 +417,15 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     const int64_t cell = (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
     const Vec3f tgt{cell_centre(g.gxmin, ix + g.ox, g.dx), cell_centre(g.gymin, iy + g.oy, g.dy),
                     cell_centre(g.gzmin, iz + g.oz, g.dz)};
-    float dens = 0, fHI = 0, fHeI = 0, fHeII = 0;
-    if (valid) {
-      dens = *mesh_f(a.mesh, cell, kOffDens);
-      fHI = *mesh_f(a.mesh, cell, kOffFHI);
-      fHeI = *mesh_f(a.mesh, cell, kOffFHeI);
-      fHeII = *mesh_f(a.mesh, cell, kOffFHeII);
+    {
+      float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (valid)
+        f = make_float4(*mesh_f(a.mesh, cell, kOffDens), *mesh_f(a.mesh, cell, kOffFHI),
+                        *mesh_f(a.mesh, cell, kOffFHeI), *mesh_f(a.mesh, cell, kOffFHeII));
+      cell_lds[lane] = f;
+#pragma unroll
+      for (int k = 0; k < 6; k++) acc_lds[k][lane] = 0.0;
     }
-    double acc[6] = {0, 0, 0, 0, 0, 0};
     int p = valid ? 0 : tr.nnode;
     int seq = 0;
     for (;;) {
@@ -447,7 +457,8 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
         }
         segment_start(g, src, tgt, &st);
         walk_segment<KAHAN, CAPTURE>(g, a.nf, st, tgt, &w);
-        ray_rate_setup(g, rt, w.tau, src, tgt, dens, fHI, fHeI, fHeII, &rr);
+        const float4 f = cell_lds[lane];
+        ray_rate_setup(g, rt, w.tau, src, tgt, f.x, f.y, f.z, f.w, &rr);
       }
       bool exact = emit;
       if constexpr (SURR) {
@@ -479,9 +490,9 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
       }
       if (emit) {
 #pragma unroll
-        for (int k = 0; k < 6; k++) acc[k] += res[k];
+        for (int k = 0; k < 6; k++) acc_lds[k][lane] += res[k];
         my_ray++;
-        my_cross += (unsigned long long)w.ncross;
+        my_cross += (unsigned int)w.ncross;
         if (CAPTURE) {
           const unsigned long long slot = atomicAdd(&a.ctr->nrecord, 1ull);
           if (slot < (unsigned long long)a.rec_cap) {
@@ -505,11 +516,13 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
     if (valid) {
       float* r = mesh_fw(a.mesh, cell, kOffRates);
 #pragma unroll
-      for (int k = 0; k < 6; k++) r[k] = (float)((double)r[k] + acc[k]);
+      for (int k = 0; k < 6; k++) r[k] = (float)((double)r[k] + acc_lds[k][lane]);
     }
   }
-  const unsigned long long sr = wave_sum(my_ray), sc = wave_sum(my_cross),
-                           sf = wave_sum(my_fb), ss = wave_sum(my_surr);
+  const unsigned long long sr = wave_sum((unsigned long long)my_ray),
+                           sc = wave_sum((unsigned long long)my_cross),
+                           sf = wave_sum((unsigned long long)my_fb),
+                           ss = wave_sum((unsigned long long)my_surr);
   if (lane == 0) {
     atomicAdd(&a.ctr->nray, sr);
     atomicAdd(&a.ctr->ncross, sc);
