@@ -104,10 +104,11 @@ __device__ inline double k06_B(double t) {
   return 1.3765e-14 * pow(lambda, 1.5) / pow(1.0 + pow(lambda / 2.740, 0.407), 2.242);
 }
 
-__global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
+__global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int nx, int ny, int nz,
                                  double denstonh, double uenetok, double helium_fact,
                                  float csHI, float csHeI, float csHeII,
-                                 float4* __restrict__ med) {
+                                 float4* __restrict__ med, float4* __restrict__ medT) {
+  const int64_t ncell = (int64_t)nx * ny * nz;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   const unsigned char* m = mesh + c * kStride;
@@ -142,6 +143,23 @@ __global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int64_t
   s.w = 0.0f;
   med[2 * c] = k;
   med[2 * c + 1] = s;
+  // the same record in the y-fastest layout of z-dominant directions
+  const int ix = (int)(c / ((int64_t)ny * nz));
+  const int iy = (int)((c / nz) % ny);
+  const int iz = (int)(c % nz);
+  const int64_t cT = (int64_t)iy + (int64_t)ny * ((int64_t)ix + (int64_t)nx * iz);
+  medT[2 * cT] = k;
+  medT[2 * cT + 1] = s;
+}
+
+// Cell index of the layout a direction's slices use: the mesh order (z
+// fastest) for directions whose dominant axis is x or y, y fastest for
+// z-dominant ones (base 0, 1), so that the rays of neighbouring lanes, which
+// start two cells apart across the entry face, touch neighbouring records.
+__device__ __forceinline__ int64_t art_index(bool zdom, int nx, int ny, int nz, int ix, int iy,
+                                             int iz) {
+  return zdom ? (int64_t)iy + (int64_t)ny * ((int64_t)ix + (int64_t)nx * iz)
+              : (int64_t)iz + (int64_t)nz * ((int64_t)iy + (int64_t)ny * ix);
 }
 
 // ----------------------------------------------------------------------
@@ -154,7 +172,8 @@ struct TraceArgs {
   int group;              // ray group 0..3 (set_boundary_ray's parity classes)
   int nray_max;           // rays per group and direction (largest face / 4)
   const float4* med;      // [2*ncell]: absorption (3 bands), source function (3 bands)
-  float4* acc;            // [ndir][ncell] {length, I1, I2, I3}
+  const float4* medT;     // the same, y-fastest layout (z-dominant directions)
+  float4* acc;            // [ndir][ncell] {length, I1, I2, I3}, in the direction's layout
   unsigned long long* ncross;
 };
 
@@ -240,11 +259,10 @@ __global__ __launch_bounds__(256) void art_trace_kernel(TraceArgs a) {
         return ix >= g.nx || iy >= g.ny || iz >= g.nz || ix < 0 || iy < 0 || iz < 0;
       };
       float I0 = 0.0f, I1 = 0.0f, I2 = 0.0f;
+      const bool zdom = base <= 1;
       float4* __restrict__ acc = a.acc + (int64_t)d * g.ncell;
-      const float4* __restrict__ med = a.med;
-      auto cell = [&]() {
-        return (int64_t)iz + (int64_t)g.nz * ((int64_t)iy + (int64_t)g.ny * ix);
-      };
+      const float4* __restrict__ med = zdom ? a.medT : a.med;
+      auto cell = [&]() { return art_index(zdom, g.nx, g.ny, g.nz, ix, iy, iz); };
       bool live = !out();
       int64_t pos = 0;
       float4 kp{}, sp{}, ac{};
@@ -339,12 +357,43 @@ __global__ void art_fill_kernel(float4* __restrict__ p, int64_t n) {
   if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
+// The slices of the batch's z-dominant directions back into mesh order
+// (acc -> out), one 32x32 (z, y) tile of one x plane per block through LDS so
+// that both the reads (y fastest) and the writes (z fastest) are contiguous.
+// Grid: (ceil(ny/32) ceil(nz/32), nx, ndir); blocks of other directions
+// return at once.
+__global__ __launch_bounds__(256) void art_relayout_kernel(const float4* __restrict__ acc,
+                                                           float4* __restrict__ out,
+                                                           const ArtDir* __restrict__ dirs,
+                                                           int nx, int ny, int nz) {
+  const int d = blockIdx.z;
+  if (dirs[d].base > 1) return;
+  __shared__ float4 tile[32][33];
+  const int ntz = (nz + 31) / 32;
+  const int y0 = (int)(blockIdx.x / ntz) * 32, z0 = (int)(blockIdx.x % ntz) * 32;
+  const int ix = blockIdx.y;
+  const int64_t ncell = (int64_t)nx * ny * nz;
+  const float4* src = acc + (int64_t)d * ncell;
+  float4* dst = out + (int64_t)d * ncell;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8) {
+    const int iz = z0 + r, iy = y0 + tx;
+    if (iz < nz && iy < ny) tile[r][tx] = src[art_index(true, nx, ny, nz, ix, iy, iz)];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int iy = y0 + r, iz = z0 + tx;
+    if (iz < nz && iy < ny) dst[art_index(false, nx, ny, nz, ix, iy, iz)] = tile[tx][r];
+  }
+}
+
 // ----------------------------------------------------------------------
 // calc_GH_tot for the batch's directions in order: gh[k][c] (FP32, 6
 // outputs GHI, HHI, GHeI, HHeI, GHeII, HHeII); optionally Kahan-compensated
 // across directions.
 template <bool KAHAN>
-__global__ void art_gh_kernel(const float4* __restrict__ acc, int ndir, int64_t ncell,
+__global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __restrict__ accm,
+                              const ArtDir* __restrict__ dirs, int ndir, int64_t ncell,
                               const float4* __restrict__ med,
                               float f0, float f1, float f2, float f3, float f4, float f5,
                               float* __restrict__ gh, float* __restrict__ comp) {
@@ -359,7 +408,9 @@ __global__ void art_gh_kernel(const float4* __restrict__ acc, int ndir, int64_t 
   }
   const double f[6] = {f0, f1, f2, f3, f4, f5};
   for (int d = 0; d < ndir; d++) {
-    const float4 a = acc[(int64_t)d * ncell + c];
+    // z-dominant directions' slices were put back into mesh order in accm
+    const float4* src = dirs[d].base <= 1 ? accm : acc;
+    const float4 a = src[(int64_t)d * ncell + c];
     const double len = a.x;
     double v[6];
     // HI
@@ -452,6 +503,7 @@ struct ArtCtx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ArtDir* d_dirs = nullptr;
   float4 *med = nullptr, *acc = nullptr;   // med: [2*ncell] absorption, source
+  float4 *medT = nullptr, *accm = nullptr; // y-fastest medium; acc slices in mesh order
   float *gh = nullptr, *comp = nullptr;
   unsigned long long* ncross = nullptr;
   int64_t ncell = 0;
@@ -657,16 +709,19 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
     for (auto& e : a_.ev) A_OK(hipEventCreate(&e));
   }
   for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
-                  (void*)a_.comp, (void*)a_.ncross})
+                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT, (void*)a_.accm})
     if (p) A_OK(hipFree(p));
   a_.d_dirs = nullptr;
   a_.med = a_.acc = nullptr;
+  a_.medT = a_.accm = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
   A_OK(hipMalloc(&a_.d_dirs, sizeof(ArtDir) * nang));
   A_OK(hipMalloc(&a_.med, sizeof(float4) * 2 * ncell));
   A_OK(hipMalloc(&a_.acc, sizeof(float4) * ncell * batch));
+  A_OK(hipMalloc(&a_.medT, sizeof(float4) * 2 * ncell));
+  A_OK(hipMalloc(&a_.accm, sizeof(float4) * ncell * batch));
   A_OK(hipMalloc(&a_.gh, sizeof(float) * 6 * ncell));
   A_OK(hipMalloc(&a_.comp, sizeof(float) * 6 * ncell));
   A_OK(hipMalloc(&a_.ncross, sizeof(unsigned long long)));
@@ -744,9 +799,9 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
   const unsigned nb = (unsigned)((nc + 255) / 256);
   A_OK(hipEventRecord(a_.ev[0], st));
   art_rmesh_kernel<<<nb, 256, 0, st>>>(
-      dmesh, nc, this_run->denstonh, this_run->uenetok, argot_hip_internal_helium_fact(),
-      (float)csect_HI(1.0 + 1.0e-7), (float)csect_HeI(1.8125 + 1.0e-7),
-      (float)csect_HeII(4.0 + 1.0e-7), a_.med);
+      dmesh, g.nx, g.ny, g.nz, this_run->denstonh, this_run->uenetok,
+      argot_hip_internal_helium_fact(), (float)csect_HI(1.0 + 1.0e-7),
+      (float)csect_HeI(1.8125 + 1.0e-7), (float)csect_HeII(4.0 + 1.0e-7), a_.med, a_.medT);
   A_OK(hipGetLastError());
   A_OK(hipMemsetAsync(a_.gh, 0, sizeof(float) * 6 * nc, st));
   A_OK(hipMemsetAsync(a_.comp, 0, sizeof(float) * 6 * nc, st));
@@ -765,6 +820,7 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
     ta.ndir = nd;
     ta.nray_max = nray_max;
     ta.med = a_.med;
+    ta.medT = a_.medT;
     ta.acc = a_.acc;
     ta.ncross = a_.ncross;
     const int64_t nthread = (int64_t)nd * nray_max;
@@ -776,13 +832,21 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
         art_trace_kernel<false><<<(unsigned)((nthread + 255) / 256), 256, 0, st>>>(ta);
     }
     A_OK(hipEventRecord(a_.ev[2], st));
+    {
+      const dim3 grid((unsigned)(((g.ny + 31) / 32) * ((g.nz + 31) / 32)), (unsigned)g.nx,
+                      (unsigned)nd);
+      art_relayout_kernel<<<grid, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, g.nx, g.ny,
+                                                g.nz);
+    }
     const float* f = a_.fact;
     if (g_kahan)
-      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.med, f[0], f[1], f[2],
-                                              f[3], f[4], f[5], a_.gh, a_.comp);
+      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, nc, a_.med,
+                                              f[0], f[1], f[2], f[3], f[4], f[5], a_.gh,
+                                              a_.comp);
     else
-      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, nd, nc, a_.med, f[0], f[1],
-                                               f[2], f[3], f[4], f[5], a_.gh, a_.comp);
+      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, nc,
+                                               a_.med, f[0], f[1], f[2], f[3], f[4], f[5],
+                                               a_.gh, a_.comp);
     A_OK(hipGetLastError());
     A_OK(hipEventRecord(a_.ev[3], st));
     A_OK(hipEventSynchronize(a_.ev[3]));
@@ -809,10 +873,11 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
 void free_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cuda) {
   (void)this_cuda;
   for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
-                  (void*)a_.comp, (void*)a_.ncross})
+                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT, (void*)a_.accm})
     if (p) (void)hipFree(p);
   a_.d_dirs = nullptr;
   a_.med = a_.acc = nullptr;
+  a_.medT = a_.accm = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
