@@ -69,6 +69,21 @@ struct ArtGrid {
 };
 
 // ----------------------------------------------------------------------
+// Position of index i along an axis of even length n with the even indices
+// first: the rays of one group start two cells apart across the entry face,
+// so with this order neighbouring lanes touch neighbouring records.
+__device__ __forceinline__ int art_split(int i, int n) { return (i & 1) * (n >> 1) + (i >> 1); }
+
+// Cell index of the layout a direction's slices use: z fastest (split) for
+// directions whose dominant axis is x or y; y fastest (split) for z-dominant
+// ones (base 0, 1), whose neighbouring rays are two cells apart in y.
+__device__ __forceinline__ int64_t art_index(bool zdom, int nx, int ny, int nz, int ix, int iy,
+                                             int iz) {
+  return zdom ? (int64_t)art_split(iy, ny) + (int64_t)ny * ((int64_t)ix + (int64_t)nx * iz)
+              : (int64_t)art_split(iz, nz) + (int64_t)nz * ((int64_t)iy + (int64_t)ny * ix);
+}
+
+
 // Per-cell absorption and source function (calc_rmesh_data.c:37-117).
 
 __device__ inline double k02_A(double t) {
@@ -141,26 +156,18 @@ __global__ void art_rmesh_kernel(const unsigned char* __restrict__ mesh, int nx,
   s.z = (float)((k06_A(t) - k06_B(t)) * ne * nHeIII * kNuLeV * 4.0 * kEVtoErg / (4.0 * kPiArt) /
                 (double)k.z);
   s.w = 0.0f;
-  med[2 * c] = k;
-  med[2 * c + 1] = s;
-  // the same record in the y-fastest layout of z-dominant directions
+  // the record in both trace layouts
   const int ix = (int)(c / ((int64_t)ny * nz));
   const int iy = (int)((c / nz) % ny);
   const int iz = (int)(c % nz);
-  const int64_t cT = (int64_t)iy + (int64_t)ny * ((int64_t)ix + (int64_t)nx * iz);
+  const int64_t cm = art_index(false, nx, ny, nz, ix, iy, iz);
+  const int64_t cT = art_index(true, nx, ny, nz, ix, iy, iz);
+  med[2 * cm] = k;
+  med[2 * cm + 1] = s;
   medT[2 * cT] = k;
   medT[2 * cT + 1] = s;
 }
 
-// Cell index of the layout a direction's slices use: the mesh order (z
-// fastest) for directions whose dominant axis is x or y, y fastest for
-// z-dominant ones (base 0, 1), so that the rays of neighbouring lanes, which
-// start two cells apart across the entry face, touch neighbouring records.
-__device__ __forceinline__ int64_t art_index(bool zdom, int nx, int ny, int nz, int ix, int iy,
-                                             int iz) {
-  return zdom ? (int64_t)iy + (int64_t)ny * ((int64_t)ix + (int64_t)nx * iz)
-              : (int64_t)iz + (int64_t)nz * ((int64_t)iy + (int64_t)ny * ix);
-}
 
 // ----------------------------------------------------------------------
 // Ray tracing (ray_tracing.c:60-394 on one rank).
@@ -357,9 +364,10 @@ __global__ void art_fill_kernel(float4* __restrict__ p, int64_t n) {
   if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// The slices of the batch's z-dominant directions back into mesh order
-// (acc -> out), one 32x32 (z, y) tile of one x plane per block through LDS so
-// that both the reads (y fastest) and the writes (z fastest) are contiguous.
+// The slices of the batch's z-dominant directions into the other directions'
+// layout (acc -> out), one 32x32 (z, y) tile of one x plane per block through
+// LDS so that both the reads (y fastest) and the writes (z fastest) are
+// contiguous runs.
 // Grid: (ceil(ny/32) ceil(nz/32), nx, ndir); blocks of other directions
 // return at once.
 __global__ __launch_bounds__(256) void art_relayout_kernel(const float4* __restrict__ acc,
@@ -393,13 +401,19 @@ __global__ __launch_bounds__(256) void art_relayout_kernel(const float4* __restr
 // across directions.
 template <bool KAHAN>
 __global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __restrict__ accm,
-                              const ArtDir* __restrict__ dirs, int ndir, int64_t ncell,
+                              const ArtDir* __restrict__ dirs, int ndir, int nx, int ny, int nz,
                               const float4* __restrict__ med,
                               float f0, float f1, float f2, float f3, float f4, float f5,
                               float* __restrict__ gh, float* __restrict__ comp) {
+  const int64_t ncell = (int64_t)nx * ny * nz;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
-  const float4 ka = med[2 * c], so = med[2 * c + 1];
+  // this cell's records: the layout of x- and y-dominant directions (the
+  // z-dominant ones were relaid into it)
+  const int cix = (int)(c / ((int64_t)ny * nz));
+  const int ciy = (int)((c / nz) % ny);
+  const int64_t cs = art_index(false, nx, ny, nz, cix, ciy, (int)(c % nz));
+  const float4 ka = med[2 * cs], so = med[2 * cs + 1];
   float s[6], e[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
@@ -408,9 +422,9 @@ __global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __re
   }
   const double f[6] = {f0, f1, f2, f3, f4, f5};
   for (int d = 0; d < ndir; d++) {
-    // z-dominant directions' slices were put back into mesh order in accm
+    // z-dominant directions' slices were relaid into accm
     const float4* src = dirs[d].base <= 1 ? accm : acc;
-    const float4 a = src[(int64_t)d * ncell + c];
+    const float4 a = src[(int64_t)d * ncell + cs];
     const double len = a.x;
     double v[6];
     // HI
@@ -840,12 +854,13 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
     }
     const float* f = a_.fact;
     if (g_kahan)
-      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, nc, a_.med,
+      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, g.nx, g.ny,
+                                              g.nz, a_.med,
                                               f[0], f[1], f[2], f[3], f[4], f[5], a_.gh,
                                               a_.comp);
     else
-      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, nc,
-                                               a_.med, f[0], f[1], f[2], f[3], f[4], f[5],
+      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, g.nx,
+                                               g.ny, g.nz, a_.med, f[0], f[1], f[2], f[3], f[4], f[5],
                                                a_.gh, a_.comp);
     A_OK(hipGetLastError());
     A_OK(hipEventRecord(a_.ev[3], st));
