@@ -51,7 +51,7 @@ def fill_art_mesh(w):
 class DiffuseStep:
     """One GPU, one rank: the diffuse transfer of a mesh."""
 
-    def __init__(self, w, nside=8, heatcool=True, batch=32, kahan=False, exp64=False):
+    def __init__(self, w, nside=8, heatcool=True, batch=0, kahan=False, exp64=False):
         L = lib()
         self.w = w
         self.nside, self.heatcool, self.batch, self.kahan = nside, heatcool, batch, kahan

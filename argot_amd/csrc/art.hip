@@ -421,51 +421,50 @@ __global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __re
     e[k] = KAHAN ? comp[k * ncell + c] : 0.0f;
   }
   const double f[6] = {f0, f1, f2, f3, f4, f5};
+  // Per-cell reciprocals: the float quotients I/kappa are exact through
+  // div_by; the double quotients become products (within an ulp of double).
+  const double rkx = 1.0 / (double)ka.x, rky = 1.0 / (double)ka.y, rkz = 1.0 / (double)ka.z;
+  // source-function terms do not depend on the direction
+  double sg0 = so.x;
+  sg0 += (double)so.y * kRjHI2;
+  sg0 += (double)so.z * kRjHI3;
+  double sh0 = so.x;
+  sh0 += (double)so.y * kRhHI2;
+  sh0 += (double)so.z * kRhHI3;
+  double sg1 = so.y;
+  sg1 += (double)so.z * kRjHeI3;
+  double sh1 = so.y;
+  sh1 += (double)so.z * kRhHeI3;
+  const double sg2 = so.z, sh2 = so.z;
   for (int d = 0; d < ndir; d++) {
     // z-dominant directions' slices were relaid into accm
     const float4* src = dirs[d].base <= 1 ? accm : acc;
     const float4 a = src[(int64_t)d * ncell + cs];
-    const double len = a.x;
+    const double rl = 1.0 / (double)a.x;
+    const double q1 = (double)div_by(a.y, rkx);   // I_1 / kappa_1 (float, as the reference)
+    const double q2 = (double)div_by(a.z, rky);
+    const double q3 = (double)div_by(a.w, rkz);
+    const double z2 = (double)a.z * rky, w3 = (double)a.w * rkz;
     double v[6];
     // HI
-    double Ig = (double)(a.y / ka.x);
-    Ig += (double)a.z * kRjHI2 / (double)ka.y;
-    Ig += (double)a.w * kRjHI3 / (double)ka.z;
-    double Ih = (double)(a.y / ka.x);
-    Ih += (double)a.z * kRhHI2 / (double)ka.y;
-    Ih += (double)a.w * kRhHI3 / (double)ka.z;
-    Ig /= len;
-    Ih /= len;
-    double sg = so.x;
-    sg += (double)so.y * kRjHI2;
-    sg += (double)so.z * kRjHI3;
-    double sh = so.x;
-    sh += (double)so.y * kRhHI2;
-    sh += (double)so.z * kRhHI3;
-    v[0] = (Ig + sg) * f[0];
-    v[1] = (Ih + sh) * f[1];
+    double Ig = q1;
+    Ig += z2 * kRjHI2;
+    Ig += w3 * kRjHI3;
+    double Ih = q1;
+    Ih += z2 * kRhHI2;
+    Ih += w3 * kRhHI3;
+    v[0] = (Ig * rl + sg0) * f[0];
+    v[1] = (Ih * rl + sh0) * f[1];
     // HeI
-    Ig = (double)(a.z / ka.y);
-    Ig += (double)a.w * kRjHeI3 / (double)ka.z;
-    Ih = (double)(a.z / ka.y);
-    Ih += (double)a.w * kRhHeI3 / (double)ka.z;
-    Ig /= len;
-    Ih /= len;
-    sg = so.y;
-    sg += (double)so.z * kRjHeI3;
-    sh = so.y;
-    sh += (double)so.z * kRhHeI3;
-    v[2] = (Ig + sg) * f[2];
-    v[3] = (Ih + sh) * f[3];
+    Ig = q2;
+    Ig += w3 * kRjHeI3;
+    Ih = q2;
+    Ih += w3 * kRhHeI3;
+    v[2] = (Ig * rl + sg1) * f[2];
+    v[3] = (Ih * rl + sh1) * f[3];
     // HeII
-    Ig = (double)(a.w / ka.z);
-    Ih = (double)(a.w / ka.z);
-    Ig /= len;
-    Ih /= len;
-    sg = so.z;
-    sh = so.z;
-    v[4] = (Ig + sg) * f[4];
-    v[5] = (Ih + sh) * f[5];
+    v[4] = (q3 * rl + sg2) * f[4];
+    v[5] = (q3 * rl + sh2) * f[5];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       if (KAHAN) {
@@ -504,7 +503,7 @@ __global__ void art_gh_sum_kernel(unsigned char* __restrict__ mesh, int64_t ncel
 
 int g_nside = 8;             // N_SIDE (run_param.h:159)
 int g_heatcool = 1;          // __HEATCOOL__ (src/Makefile:187)
-int g_batch = 32;            // directions per batch
+int g_batch = 0;             // directions per batch; 0: as many as 8 GiB of slices hold, <= 128
 int g_kahan = 0;
 int g_exp64 = 0;             // 1: exp/expm1 in FP64 as the reference
 argot_diffuse_stats g_dstats{};
@@ -675,7 +674,7 @@ extern "C" {
 
 void argot_hip_set_diffuse_nside(int nside) { g_nside = nside > 0 ? nside : 8; }
 void argot_hip_set_diffuse_heatcool(int on) { g_heatcool = on ? 1 : 0; }
-void argot_hip_set_diffuse_batch(int ndir) { g_batch = ndir > 0 ? ndir : 32; }
+void argot_hip_set_diffuse_batch(int ndir) { g_batch = ndir > 0 ? ndir : 0; }
 void argot_hip_set_diffuse_kahan(int on) { g_kahan = on ? 1 : 0; }
 void argot_hip_set_diffuse_exp64(int on) { g_exp64 = on ? 1 : 0; }
 void argot_hip_diffuse_stats(struct argot_diffuse_stats* out) {
@@ -717,7 +716,14 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
     return argot_hip_internal_set_error(ARGOT_ERR_ARG);   // ray groups need even sizes
   const int64_t ncell = (int64_t)nx * ny * nz;
   const int nang = 12 * g_nside * g_nside;
-  const int batch = std::min(g_batch, nang);
+  // Each direction of a batch owns two float4 slices (trace layout and
+  // relaid); more directions per batch keep more rays in flight.
+  int batch = g_batch;
+  if (batch <= 0) {
+    const int64_t fit = ((int64_t)8 << 30) / (32 * ncell);
+    batch = (int)std::max<int64_t>(1, std::min<int64_t>(128, fit));
+  }
+  batch = std::min(batch, nang);
   if (!a_.stream) {
     A_OK(hipStreamCreate(&a_.stream));
     for (auto& e : a_.ev) A_OK(hipEventCreate(&e));

@@ -150,7 +150,7 @@ def art_leg():
     from argot_amd import art as A
     from argot_amd import ic
     w = ic.art128()
-    st = A.DiffuseStep(w, nside=8, batch=32)
+    st = A.DiffuseStep(w, nside=8)
     st.step()
     t0 = time.perf_counter()
     s = st.step()

@@ -415,7 +415,9 @@ void free_diffuse_gpu(struct cuda_diffuse_param *cd_param, struct cuda_param *th
 /* ART controls (no reference counterpart): N_SIDE (reference: compile-time
  * 8, run_param.h:159; takes effect at init_hd_param/setup/init), the
  * __HEATCOOL__ step factors (default on, src/Makefile:187), directions
- * traced per batch, Kahan-compensated sums over directions. */
+ * traced per batch (0, the default: as many as 8 GiB of accumulator slices
+ * hold, at most 128; read by init_diffuse_gpu), Kahan-compensated sums over
+ * directions. */
 void argot_hip_set_diffuse_nside(int nside);
 void argot_hip_set_diffuse_heatcool(int on);
 void argot_hip_set_diffuse_batch(int ndir);
