@@ -826,7 +826,6 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
   A_OK(hipMemsetAsync(a_.gh, 0, sizeof(float) * 6 * nc, st));
   A_OK(hipMemsetAsync(a_.comp, 0, sizeof(float) * 6 * nc, st));
   A_OK(hipMemsetAsync(a_.ncross, 0, sizeof(unsigned long long), st));
-  A_OK(hipEventRecord(a_.ev[1], st));
   const int nang = (int)a_.dirs.size();
   const int nray_max = std::max(g.nx * g.ny, std::max(g.ny * g.nz, g.nz * g.nx)) / 4;
   float trace_ms = 0.0f, gh_ms = 0.0f;
@@ -834,6 +833,7 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
     const int nd = std::min(a_.batch, nang - d0);
     const int64_t nacc = (int64_t)nd * nc;
     art_fill_kernel<<<(unsigned)((nacc + 255) / 256), 256, 0, st>>>(a_.acc, nacc);
+    A_OK(hipEventRecord(a_.ev[1], st));
     TraceArgs ta;
     ta.g = g;
     ta.dirs = a_.d_dirs + d0;
@@ -876,7 +876,6 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
     A_OK(hipEventElapsedTime(&m1, a_.ev[2], a_.ev[3]));
     trace_ms += m0;
     gh_ms += m1;
-    A_OK(hipEventRecord(a_.ev[1], st));
   }
   const float r_n_ang = (float)(1.0e0 / nang);
   art_gh_sum_kernel<<<nb, 256, 0, st>>>(dmesh, nc, a_.gh, r_n_ang);

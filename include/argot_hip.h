@@ -429,7 +429,7 @@ void argot_hip_set_diffuse_exp64(int on);
 struct argot_diffuse_stats {
     uint64_t ndir;           /* directions */
     uint64_t ncross;         /* ray-cell crossings */
-    double ms_trace;         /* ray tracing kernels, HIP events */
+    double ms_trace;         /* ray tracing kernels only (not the slice fill), HIP events */
     double ms_gh;            /* per-direction rate kernels, HIP events */
     double ms_total;         /* whole call, wall clock */
 };
