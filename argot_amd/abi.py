@@ -121,7 +121,7 @@ class StepStats(C.Structure):
                 ("nfallback", C.c_uint64), ("ms_build", C.c_double), ("ms_kernel", C.c_double),
                 ("ms_total", C.c_double), ("nsurrogate", C.c_uint64),
                 ("iterations", C.c_uint64), ("ms_chem", C.c_double),
-                ("nvisit", C.c_uint64), ("nemit_iter", C.c_uint64)]
+                ("nvisit", C.c_uint64), ("nemit_iter", C.c_uint64), ("ms_walk", C.c_double)]
 
 
 class ChemStats(C.Structure):
@@ -212,3 +212,23 @@ TUNE_MAX = 32
 
 class TuneReport(C.Structure):
     _fields_ = [("n", C.c_int), ("best", C.c_int), ("entry", TuneEntry * TUNE_MAX)]
+
+
+_NP_SCALAR = {C.c_uint64: np.uint64, C.c_int64: np.int64, C.c_int: np.int32, C.c_uint: np.uint32,
+              C.c_float: np.float32, C.c_double: np.float64}
+
+
+def flat_dtype(ctype):
+    """numpy dtype with the field offsets and size of a ctypes structure of
+    scalar fields (ctypes' own buffer export guesses the tail padding)."""
+    names = [f for f, _ in ctype._fields_]
+    return np.dtype({"names": names,
+                     "formats": [_NP_SCALAR[t] for _, t in ctype._fields_],
+                     "offsets": [getattr(ctype, f).offset for f in names],
+                     "itemsize": C.sizeof(ctype)})
+
+
+def view_at(ctype, address, count):
+    """numpy structured view (no copy) of count structures at address."""
+    buf = (C.c_char * (C.sizeof(ctype) * count)).from_address(address)
+    return np.frombuffer(buf, dtype=flat_dtype(ctype))

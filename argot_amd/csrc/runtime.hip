@@ -421,9 +421,14 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
   HIP_OK(g.seg.reserve(bytes));
   HIP_OK(hipMemcpyAsync(g.seg.p, seg, bytes, hipMemcpyHostToDevice, g.stream));
   const GridDev gd = make_grid(this_run);
+  HIP_OK(hipEventRecord(g.ev_c0, g.stream));
   HIP_OK(launch_walk_segments(gd, g.nf.p, g.seg.p, nseg, g_tau_mode, g.stream));
+  HIP_OK(hipEventRecord(g.ev_c1, g.stream));
   HIP_OK(hipMemcpyAsync(seg, g.seg.p, bytes, hipMemcpyDeviceToHost, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, g.ev_c0, g.ev_c1));
+  g.stats.ms_walk = ms;
 }
 
 void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* ray,

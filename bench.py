@@ -108,6 +108,62 @@ def surrogate_leg(step, plain_seg_s, nray):
     return out
 
 
+def walk_leg(step, nsample=8 << 20):
+    """Walk only: calc_optical_depth_cuda (the reference's walk entry point:
+    56-B ray_segment records, one lane per segment) on the first nsample
+    segments the step emits, captured from the fused kernel.  Their optical
+    depths must equal the fused kernel's bit for bit."""
+    import ctypes as C
+    from argot_amd import abi
+    from argot_amd._lib import check, lib
+    rec, _ = step.capture(nsample)
+    n = len(rec)
+    side = step.w.n
+    run = step.run
+    t = rec["target"]
+    idx = np.stack([t // (side * side), (t // side) % side, t % side], axis=1)
+    lo = np.array([run.xmin, run.ymin, run.zmin], np.float32).astype(np.float64)
+    h = np.array([run.delta_x, run.delta_y, run.delta_z], np.float32).astype(np.float64)
+    # target centre as the kernels form it: lo + ((float)i + 0.5) h in
+    # double, rounded to float
+    centre = (lo + (idx.astype(np.float32).astype(np.float64) + 0.5) * h).astype(np.float32)
+    L = lib()
+    segp = C.c_void_p()
+    L.allocate_pinned_segment(C.byref(segp), n)
+    check("allocate_pinned_segment")
+    try:
+        seg = abi.view_at(abi.RaySegment, segp.value, n)
+        seg["ray_indx"] = np.arange(n, dtype=np.uint64)
+        for k, a in enumerate("xyz"):
+            seg[f"{a}pos_start"] = rec["start"][:, k]
+            seg[f"{a}pos_end"] = centre[:, k]
+        run.nseg = n
+        ms = []
+        for _ in range(3):
+            L.calc_optical_depth_cuda(segp, C.byref(step.cmem), C.byref(step.cpar), C.byref(run))
+            check("calc_optical_depth_cuda")
+            ms.append(step.stats()["ms_walk"])
+        tau = np.stack([seg["optical_depth_HI"], seg["optical_depth_HeI"],
+                        seg["optical_depth_HeII"]], axis=1)
+        same = bool(np.array_equal(tau.view(np.uint32), rec["tau"].view(np.uint32)))
+    finally:
+        L.deallocate_pinned_segment(segp)
+    ncross = int(rec["ncross"].astype(np.int64).sum())
+    kms = float(np.mean(ms[1:]))
+    # per segment: the 56-B record read, 12 B of optical depths written;
+    # 12 B gathered per crossing
+    bytes_ = n * (56 + 12) + 12 * ncross
+    out = {"entry_point": "calc_optical_depth_cuda", "segments": n, "crossings": ncross,
+           "ms_kernel": kms, "segments_per_s": n / (kms * 1e-3),
+           "crossings_per_s": ncross / (kms * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": bytes_ / (kms * 1e-3) / 1e9,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": bytes_ / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+           "tau_equal_to_fused_kernel": same}
+    log(f"walk only: {out}")
+    return out
+
+
 def chem_leg(step, dtime=1.0e-3):
     """Chemistry of the whole mesh over dtime (code units) after the pass, as
     the reference's loop runs it: smoothed rates, then step_chemistry; run
@@ -184,10 +240,12 @@ def main():
                     help="run only the ART leg and print its JSON (profiling)")
     ap.add_argument("--chem-only", action="store_true",
                     help="run one point-source pass and the chemistry leg, print its JSON")
+    ap.add_argument("--walk-only", action="store_true",
+                    help="run the walk-only leg (calc_optical_depth_cuda), print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the library drives one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
         sys.exit(2)
     if args.art_only:
         print(json.dumps(art_leg()), flush=True)
@@ -204,6 +262,9 @@ def main():
     if args.chem_only:
         step.step()
         print(json.dumps(chem_leg(step)), flush=True)
+        return
+    if args.walk_only:
+        print(json.dumps(walk_leg(step)), flush=True)
         return
 
     for i in range(args.warmup):
@@ -284,6 +345,7 @@ def main():
     }
     if not args.no_extras:
         out["surrogate"] = surrogate_leg(step, seg_s, nray)
+        out["walk_only"] = walk_leg(step)
         out["chemistry"] = chem_leg(step)
         out["art"] = art_leg()
         log(f"art: {out['art']}")

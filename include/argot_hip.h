@@ -485,6 +485,9 @@ struct argot_step_stats {
     uint64_t nemit_iter;     /* visits in which at least one lane emitted a ray:
                                 nray / (64 nemit_iter) is the lane utilisation of
                                 the walk and the rate loop */
+    double ms_walk;          /* walk kernel of the last calc_optical_depth_cuda
+                                call (segment list already on the device), HIP
+                                events */
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
 

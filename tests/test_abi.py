@@ -68,3 +68,20 @@ def test_host_tree_handles_tiny_inputs():
     t = np.array([0.1, 0.2, 0.3], np.float32)
     assert L.argot_hip_tree_rays_to(h, t.ctypes.data, 0, None, None, None) == 1
     L.argot_hip_tree_free(h)
+
+
+def test_flat_dtype_views_match_ctypes():
+    # the numpy view of a ctypes array reads and writes the same bytes,
+    # tail padding included (RaySegment: 52 B of fields in a 56-B record)
+    import ctypes as C
+    a = (abi.RaySegment * 4)()
+    v = abi.view_at(abi.RaySegment, C.addressof(a), 4)
+    assert v.dtype.itemsize == C.sizeof(abi.RaySegment) == 56
+    v["ray_indx"] = [1, 2, 3, 2**63 + 5]
+    v["zpos_end"] = [0.5, 1.5, 2.5, 3.5]
+    v["optical_depth_HeII"] = [7.0, 8.0, 9.0, 10.0]
+    assert [r.ray_indx for r in a] == [1, 2, 3, 2**63 + 5]
+    assert [r.zpos_end for r in a] == [0.5, 1.5, 2.5, 3.5]
+    a[2].optical_depth_HI = 4.25
+    assert v["optical_depth_HI"][2] == np.float32(4.25)
+    assert list(v["optical_depth_HeII"]) == [7.0, 8.0, 9.0, 10.0]

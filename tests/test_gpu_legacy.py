@@ -9,7 +9,7 @@ import pytest
 from argot_amd import abi
 from argot_amd import radiation as R
 from argot_amd._lib import check, lib
-from helpers import oracle_problem, rel_err, small_workload
+from helpers import cell_centre, oracle_problem, rel_err, small_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +126,35 @@ def test_merge_photoion_data(require_gpu):
     check("merge_photoion_data")
     got = step.rates()
     assert np.array_equal(got, add + add)
+
+
+def test_walk_entry_point_equals_fused_walk(require_gpu):
+    # every segment the fused step emits, walked again through
+    # calc_optical_depth_cuda (one lane per segment): the same optical-depth
+    # bits, and the kernel time is reported
+    w = small_workload(n=32, nsrc=16, seed=31)
+    step = R.PointSourceStep(w)
+    rec, st = step.capture(1500000)
+    n = len(rec)
+    assert n == st["nray"] > 0
+    side = w.n
+    t = rec["target"]
+    idx = np.stack([t // (side * side), (t // side) % side, t % side], axis=1)
+    L = lib()
+    segp = C.c_void_p()
+    L.allocate_pinned_segment(C.byref(segp), n)
+    check("allocate_pinned_segment")
+    try:
+        seg = abi.view_at(abi.RaySegment, segp.value, n)
+        for k, a in enumerate("xyz"):
+            seg[f"{a}pos_start"] = rec["start"][:, k]
+            seg[f"{a}pos_end"] = cell_centre(idx[:, k], side)
+        step.run.nseg = n
+        L.calc_optical_depth_cuda(segp, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+        check("calc_optical_depth_cuda")
+        tau = np.stack([seg["optical_depth_HI"], seg["optical_depth_HeI"],
+                        seg["optical_depth_HeII"]], axis=1).copy()
+    finally:
+        L.deallocate_pinned_segment(segp)
+    assert np.array_equal(tau.view(np.uint32), rec["tau"].view(np.uint32))
+    assert step.stats()["ms_walk"] > 0.0
