@@ -84,25 +84,35 @@ __global__ void walk_segments_kernel(GridDev g, const float4* __restrict__ nf,
 // species s >= NS have sigma_s = 0 in these bins, so their optical depth
 // enters only through the constant cst).  Tables are read with scalar loads
 // (wave-uniform bin index); sums run in FP32 over 32 bins, then in FP64.
-// sinh(y)/y = sum_k y^2k/(2k+1)! through y^14: truncation < 1e-10 relative
-// for y <= 2.
-__device__ __forceinline__ float sinhc_small(float y) {
-  const float y2 = y * y;
-  float p = __builtin_fmaf(y2, 1.0f / 1307674368000.0f, 1.0f / 6227020800.0f);
-  p = __builtin_fmaf(y2, p, 1.0f / 39916800.0f);
-  p = __builtin_fmaf(y2, p, 1.0f / 362880.0f);
-  p = __builtin_fmaf(y2, p, 1.0f / 5040.0f);
-  p = __builtin_fmaf(y2, p, 1.0f / 120.0f);
-  p = __builtin_fmaf(y2, p, 1.0f / 6.0f);
-  return __builtin_fmaf(y2, p, 1.0f);
+constexpr float kLog2e = 1.4426950408889634f;
+
+// sinh(y)/y for 0 <= y <= 2 from yl = -log2(e) y: a degree-5 polynomial in
+// u = y^2 (weighted least-squares fit over [0, 4], relative error 3e-10
+// before rounding), evaluated in yl^2 with coefficient k scaled by
+// log2(e)^(-2k).
+constexpr double kL2 = 1.4426950408889634 * 1.4426950408889634;
+constexpr float kSh1 = (float)(1.6666667196e-01 / kL2);
+constexpr float kSh2 = (float)(8.3333171799e-03 / (kL2 * kL2));
+constexpr float kSh3 = (float)(1.9843047992e-04 / (kL2 * kL2 * kL2));
+constexpr float kSh4 = (float)(2.7469777882e-06 / (kL2 * kL2 * kL2 * kL2));
+constexpr float kSh5 = (float)(2.7015661901e-08 / (kL2 * kL2 * kL2 * kL2 * kL2));
+__device__ __forceinline__ float sinhc_l(float yl) {
+  const float u = yl * yl;
+  float p = __builtin_fmaf(u, kSh5, kSh4);
+  p = __builtin_fmaf(u, p, kSh3);
+  p = __builtin_fmaf(u, p, kSh2);
+  p = __builtin_fmaf(u, p, kSh1);
+  return __builtin_fmaf(u, p, 1.0f);
 }
 
 // Bins [k0, k1) of a ray whose in-cell optical depth is at most 4 in every
 // bin: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y with
-// y = dtau/2, one exponential per bin.
+// y = dtau/2, one exponential per bin.  dl holds the ray's in-cell weights
+// times -log2(e)/2, so that their sum with the cross sections is
+// yl = -log2(e) y and 2^(yl - log2(e) tau) = exp(-(tau + y)).
 template <int NS, typename SpecF>
 __device__ __forceinline__ void rate_bins_thin(const RateTables& rt, int k0, int k1, float cst,
-                                               const float n[3], const float d[3], SpecF spec,
+                                               const float n[3], const float dl[3], SpecF spec,
                                                double G[3], double H[3]) {
   const int ng = rt.ngrid;
   for (int c = k0; c < k1; c += 32) {
@@ -113,16 +123,16 @@ __device__ __forceinline__ void rate_bins_thin(const RateTables& rt, int k0, int
 #pragma unroll 4
     for (int k = c; k < ce; k++) {
       float sg[NS], hs[NS];
-      float tau = cst, dt = 0.0f;
+      float tau = cst, yl = 0.0f;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         sg[s] = ldc(rt.sig + s * ng + k);
         hs[s] = ldc(rt.hsig + s * ng + k);
         tau = __builtin_fmaf(n[s], ldc(rt.dsig + s * ng + k), tau);
-        dt = __builtin_fmaf(d[s], sg[s], dt);
+        yl = __builtin_fmaf(dl[s], sg[s], yl);
       }
-      const float y = 0.5f * dt;
-      const float b = spec(k) * __expf(-(tau + y)) * sinhc_small(y);
+      const float e = __builtin_amdgcn_exp2f(__builtin_fmaf(tau, -kLog2e, yl));
+      const float b = spec(k) * e * sinhc_l(yl);
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         g[s] = __builtin_fmaf(sg[s], b, g[s]);
@@ -200,9 +210,12 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   // run both loops under their masks.
   const double dmax = rr.d[0] * rt.cs_max[0] + rr.d[1] * rt.cs_max[1] + rr.d[2] * rt.cs_max[2];
   if (dmax <= 4.0) {
-    rate_bins_thin<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
-    rate_bins_thin<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
-    rate_bins_thin<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
+    constexpr double kHalfLog2e = 0.5 * 1.4426950408889634;
+    const float dl[3] = {(float)(rr.d[0] * -kHalfLog2e), (float)(rr.d[1] * -kHalfLog2e),
+                         (float)(rr.d[2] * -kHalfLog2e)};
+    rate_bins_thin<1>(rt, 0, rt.k_he1, c1, n, dl, spec, G, H);
+    rate_bins_thin<2>(rt, rt.k_he1, rt.k_he2, c2, n, dl, spec, G, H);
+    rate_bins_thin<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, dl, spec, G, H);
   } else {
     rate_bins<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
     rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
