@@ -124,15 +124,23 @@ __device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3
   return nseg;
 }
 
-// Per-ray state of the DDA walk, calc_optical_depth.c:76-143.
+// Per-ray state of the DDA walk, calc_optical_depth.c:76-143.  Along each
+// axis the walk keeps, instead of the cell index i, the face counter
+// f = sign * ((float)i + step) (step = 1 when moving up, else 0; sign = +-1
+// the direction), an integer-valued float that grows by 1 at every crossing
+// of that axis: sign * delta * f == delta * ((float)i + step) exactly, and
+// the cell is inside the mesh on that axis while f < lim (lim = n + 1 when
+// moving up, 1 when moving down: the walk never moves back).  The linear
+// cell index moves by the axis stride at each crossing.
 struct Dda {
-  int ix, iy, iz;        // current cell
-  int ixe, iye, ize;     // end cell (clamped)
+  float fx, fy, fz;      // face counters
+  float sdx, sdy, sdz;   // sign * delta
+  float lx, ly, lz;      // inside while f < l
+  int stx, sty, stz;     // signed strides of the linear index
+  uint32_t idx;          // linear index of the current cell
   uint32_t idxe;         // linear index of the end cell
   float xo, yo, zo;      // direction cosines (with the TINY guards)
   double rxo, ryo, rzo;  // 1/xo etc. in double (exact-division helper)
-  float xsf, ysf, zsf;   // x_step etc. as float (0 or 1)
-  int xn, yn, zn;        // +-1
   float xc, yc, zc;      // position in local coordinates
 };
 
@@ -151,16 +159,17 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   int ixe = (int)((e.x - g.xmin) / g.dx);
   int iye = (int)((e.y - g.ymin) / g.dy);
   int ize = (int)((e.z - g.zmin) / g.dz);
-  d->ixe = max(min(ixe, g.nx - 1), 0);
-  d->iye = max(min(iye, g.ny - 1), 0);
-  d->ize = max(min(ize, g.nz - 1), 0);
+  ixe = max(min(ixe, g.nx - 1), 0);
+  iye = max(min(iye, g.ny - 1), 0);
+  ize = max(min(ize, g.nz - 1), 0);
   int ix = (int)((s.x - g.xmin) / g.dx);
   int iy = (int)((s.y - g.ymin) / g.dy);
   int iz = (int)((s.z - g.zmin) / g.dz);
-  d->ix = max(min(ix, g.nx - 1), 0);
-  d->iy = max(min(iy, g.ny - 1), 0);
-  d->iz = max(min(iz, g.nz - 1), 0);
-  d->idxe = cell_index(g, d->ixe, d->iye, d->ize);
+  ix = max(min(ix, g.nx - 1), 0);
+  iy = max(min(iy, g.ny - 1), 0);
+  iz = max(min(iz, g.nz - 1), 0);
+  d->idxe = cell_index(g, ixe, iye, ize);
+  d->idx = cell_index(g, ix, iy, iz);
 
   float dx = e.x - s.x, dy = e.y - s.y, dz = e.z - s.z;
   float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kWalkTiny);
@@ -182,40 +191,44 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->ryo = 1.0 / (double)yo;
   d->rzo = 1.0 / (double)zo;
   const bool px = (double)xo > 0.0, py = (double)yo > 0.0, pz = (double)zo > 0.0;
-  d->xsf = px ? 1.0f : 0.0f;
-  d->ysf = py ? 1.0f : 0.0f;
-  d->zsf = pz ? 1.0f : 0.0f;
-  d->xn = px ? 1 : -1;
-  d->yn = py ? 1 : -1;
-  d->zn = pz ? 1 : -1;
+  d->fx = px ? (float)ix + 1.0f : -(float)ix;
+  d->fy = py ? (float)iy + 1.0f : -(float)iy;
+  d->fz = pz ? (float)iz + 1.0f : -(float)iz;
+  d->sdx = px ? g.dx : -g.dx;
+  d->sdy = py ? g.dy : -g.dy;
+  d->sdz = pz ? g.dz : -g.dz;
+  d->lx = px ? (float)(g.nx + 1) : 1.0f;
+  d->ly = py ? (float)(g.ny + 1) : 1.0f;
+  d->lz = pz ? (float)(g.nz + 1) : 1.0f;
+  const int sz = 1, sy = g.nz, sx = g.ny * g.nz;
+  d->stx = px ? sx : -sx;
+  d->sty = py ? sy : -sy;
+  d->stz = pz ? sz : -sz;
   d->xc = s.x - g.xmin;
   d->yc = s.y - g.ymin;
   d->zc = s.z - g.zmin;
 }
 
 // The walk continues while the current cell lies inside the mesh and is not
-// the end cell; *idx receives the current cell's linear index (meaningful
-// only when the cell is inside).
-__device__ __forceinline__ bool dda_active(const GridDev& g, const Dda& d, uint32_t* idx) {
-  const bool inside = ((unsigned)d.ix < (unsigned)g.nx) & ((unsigned)d.iy < (unsigned)g.ny) &
-                      ((unsigned)d.iz < (unsigned)g.nz);
-  *idx = cell_index(g, d.ix, d.iy, d.iz);
-  return inside & (*idx != d.idxe);
+// the end cell (the index is meaningful only when the cell is inside).
+__device__ __forceinline__ bool dda_active(const Dda& d) {
+  return (d.fx < d.lx) & (d.fy < d.ly) & (d.fz < d.lz) & (d.idx != d.idxe);
 }
 
 // One step of calc_optical_depth.c:163-179; returns the chord length.
-__device__ __forceinline__ float dda_step(const GridDev& g, Dda* d) {
-  const float ddx = g.dx * ((float)d->ix + d->xsf) - d->xc;
-  const float ddy = g.dy * ((float)d->iy + d->ysf) - d->yc;
-  const float ddz = g.dz * ((float)d->iz + d->zsf) - d->zc;
+__device__ __forceinline__ float dda_step(Dda* d) {
+  const float ddx = d->sdx * d->fx - d->xc;
+  const float ddy = d->sdy * d->fy - d->yc;
+  const float ddz = d->sdz * d->fz - d->zc;
   const float rx = div_by(ddx, d->rxo);
   const float ry = div_by(ddy, d->ryo);
   const float rz = div_by(ddz, d->rzo);
   const float rmin = fminf(rx, fminf(ry, rz));
   const bool ax = rmin == rx, ay = rmin == ry, az = rmin == rz;
-  d->ix += ax ? d->xn : 0;
-  d->iy += ay ? d->yn : 0;
-  d->iz += az ? d->zn : 0;
+  d->fx += ax ? 1.0f : 0.0f;
+  d->fy += ay ? 1.0f : 0.0f;
+  d->fz += az ? 1.0f : 0.0f;
+  d->idx += (uint32_t)((ax ? d->stx : 0) + (ay ? d->sty : 0) + (az ? d->stz : 0));
   d->xc = d->xc + rmin * d->xo;
   d->yc = d->yc + rmin * d->yo;
   d->zc = d->zc + rmin * d->zo;
@@ -257,15 +270,14 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
-  uint32_t idx;
-  bool act = dda_active(g, d, &idx);
-  float4 n = nf[act ? idx : 0u];
+  bool act = dda_active(d);
+  float4 n = nf[act ? d.idx : 0u];
   while (act) {
-    if (HASH) h = (h ^ (uint64_t)idx) * 0x100000001b3ull;
+    if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
     nc++;
-    const float rmin = dda_step(g, &d);
-    const bool nact = dda_active(g, d, &idx);
-    const float4 nn = nf[nact ? idx : 0u];
+    const float rmin = dda_step(&d);
+    const bool nact = dda_active(d);
+    const float4 nn = nf[nact ? d.idx : 0u];
     if (KAHAN) {
       kahan_add(t0, c0, rmin * n.x);
       kahan_add(t1, c1, rmin * n.y);
