@@ -1,6 +1,7 @@
 """ART diffuse transfer on the GPU against the oracle: the whole pass (rates
-of every cell, crossing count) for several direction sets and mesh sizes,
-batch-size independence, and the Kahan sums over directions."""
+of every cell, crossing count) for several direction sets and mesh sizes up
+to 64^3 with 768 directions, batch-size independence (also on the full
+128^3 mesh), and the Kahan sums over directions."""
 import numpy as np
 import pytest
 
@@ -65,3 +66,33 @@ def test_kahan_sums_over_directions(require_gpu):
     ref, _ = oracle_rates(w, 8)
     assert rel(rk, ref).max() < 1e-5
     assert not np.array_equal(rk, rp)
+
+
+def test_default_batch_matches_oracle_at_64(require_gpu):
+    # the bench's settings (768 directions, automatic batch) on a 64^3 mesh
+    w = ic.art128(seed=14, n=64)
+    st = A.DiffuseStep(w, nside=8)
+    stats = st.step()
+    got = st.rates()
+    st.close()
+    ref, ncross = oracle_rates(w, 8)
+    assert stats["ndir"] == 768 and stats["ncross"] == ncross
+    assert np.isfinite(got).all()
+    assert rel(got, ref).max() < 1e-5
+
+
+def test_full_size_batches_agree(require_gpu):
+    # 128^3 x 768 directions: the automatic batch (128 directions) and
+    # batches of 32 give the same rates bit for bit and the same crossings
+    w = ic.art128()
+    a = A.DiffuseStep(w, nside=8)
+    sa = a.step()
+    ra = a.rates()
+    a.close()
+    b = A.DiffuseStep(w, nside=8, batch=32)
+    sb = b.step()
+    rb = b.rates()
+    b.close()
+    assert sa["ncross"] == sb["ncross"] > 0
+    assert np.isfinite(ra).all() and (ra >= 0).all()
+    np.testing.assert_array_equal(ra, rb)
