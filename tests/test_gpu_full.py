@@ -5,7 +5,8 @@ the mesh and of four more at source cells is compared with the oracle
 (relative 1e-5, the same inf/NaN pattern); the whole mesh is checked for
 what holds at any size: a rerun is bit-identical, finite rates are
 non-negative, and non-finite ones (rays whose source sits at the target
-centre) are confined to a few cells."""
+centre) are confined to a few cells.  The chemistry of the whole clumped256
+mesh is checked on sampled cells against the oracle."""
 import numpy as np
 import pytest
 
@@ -52,3 +53,49 @@ def test_full_size_step(require_gpu, name, kahan):
         assert np.array_equal(f, np.isfinite(got)), f"targets [{t0}, {t1})"
         err = rel_err(got[f], ref[f], RATE_FLOOR)
         assert err.max() < RATE_TOL, f"targets [{t0}, {t1}): max rel err {err.max():.3e}"
+
+
+def test_full_size_chemistry(require_gpu):
+    """clumped256 after the point-source pass (the source cells' infinite
+    rates set to zero): step_chemistry over 100 kyr on the whole mesh; 16384
+    sampled cells run through the oracle from the same records."""
+    import oracle as O
+    from argot_amd import abi
+    w = ic.clumped256()
+    st = R.PointSourceStep(w)
+    st.step()
+    st.mesh_bytes()
+    v = abi.mesh_view(st.mesh)
+    pc = v["prev_chem"]
+    for f in R.RATE_FIELDS:
+        x = pc[f]
+        x[~np.isfinite(x)] = 0.0
+        pc[f] = x
+    v["prev_chem"] = pc
+    v["chem"] = pc
+    st.send_mesh()
+    before = np.frombuffer(st.mesh, np.uint8).reshape(-1, 156)
+    rng = np.random.default_rng(5)
+    cells = np.sort(rng.choice(before.shape[0], 16384, replace=False))
+    ref = np.ascontiguousarray(before[cells])
+    dtime = np.float32(1.0e-3)
+    st.chemistry(dtime)
+    cs = st.chem_stats()
+    assert cs["ncell"] == w.n ** 3 and cs["nstopped"] == 0
+    got = np.frombuffer(st.mesh_bytes(), np.uint8).reshape(-1, 156)[cells]
+    r = st.run
+    p = O.ChemParams(r.lunit, r.munit, r.tunit, r.uenetok, r.denstonh, w.helium_fact,
+                     r.anow, r.znow)
+    O.chem_step_mesh(ref, p, dtime)
+    dt = abi.mesh_view(abi.mesh_array(1)).dtype
+    g, want = np.frombuffer(got.tobytes(), dt), np.frombuffer(ref.tobytes(), dt)
+    exact = np.ones(len(cells), bool)
+    for f in ("felec", "fHI", "fHII", "fHeI", "fHeII", "fHeIII", "wmol"):
+        a, b = g["chem"][f], want["chem"][f]
+        exact &= np.ascontiguousarray(a).view(np.uint32) == np.ascontiguousarray(b).view(np.uint32)
+        np.testing.assert_allclose(a, b, rtol=2e-4, atol=1e-12, err_msg=f)
+    a, b = g["uene"], want["uene"]
+    exact &= np.ascontiguousarray(a).view(np.uint32) == np.ascontiguousarray(b).view(np.uint32)
+    np.testing.assert_allclose(a, b, rtol=2e-5, err_msg="uene")
+    print(f"{exact.mean():.4f} of {len(cells)} sampled cells bit-exact", flush=True)
+    assert exact.mean() >= 0.9
