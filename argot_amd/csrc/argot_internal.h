@@ -63,7 +63,6 @@ struct TreeDev {
   const NodeLink* link;      // [nnode]
   const float* spec;         // [nnode*ngrid] spectrum / spec_scale, FP32
   const double* spec_scale;  // [nnode] max over bins of the spectrum
-  const double* spec64;      // [nnode*ngrid] the spectrum itself
   const float* src_pos;      // [4*nsrc]
 };
 

@@ -495,8 +495,11 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
           const float* sp = tr.spec + (int64_t)u * rt.ngrid;
           rate_loop_f32(rt, rr, ldc(tr.spec_scale + u), [&](int k) { return ldc(sp + k); }, res);
         } else {
-          const double* sp = tr.spec64 + (int64_t)u * rt.ngrid;
-          rate_loop_f64(rt, rr, [&](int k) { return sp[k]; }, res);
+          // the node spectrum as spec * spec_scale (within 6e-8 of the
+          // double sums; this loop serves the rays FP32 cannot)
+          const float* sp = tr.spec + (int64_t)u * rt.ngrid;
+          const double sc = ldc(tr.spec_scale + u);
+          rate_loop_f64(rt, rr, [&](int k) { return (double)ldc(sp + k) * sc; }, res);
           my_fb++;
         }
       }

@@ -102,7 +102,7 @@ struct Ctx {
   // tree
   DevBuf<NodeLink> link;
   DevBuf<float> cm, spec, src_pos;
-  DevBuf<double> spec_scale, spec64;
+  DevBuf<double> spec_scale;
   DevBuf<StepCounters> ctr;
   DevBuf<double> node_q;          // surrogate: photon rate of conforming nodes
   DevBuf<RayRecord> rec;
@@ -536,26 +536,29 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   if (nsrc < 0 || (nsrc > 0 && !src)) return set_err(ARGOT_ERR_ARG);
   this_run->theta_crit = g_theta;
 
-  // Source tree on the host (construct_tree), uploaded in pre-order.
+  // Source tree on the host (construct_tree), uploaded in pre-order.  The
+  // spectra are read where they lie in the radiation_src records.
+  static_assert(sizeof(struct radiation_src) % sizeof(double) == 0 &&
+                    offsetof(struct radiation_src, photon_rate) % sizeof(double) == 0,
+                "photon_rate rows are a whole number of doubles apart");
   std::vector<float> pos(3 * std::max<int64_t>(nsrc, 1));
-  std::vector<double> spec((size_t)ARGOT_NGRID_NU * std::max<int64_t>(nsrc, 1));
   for (int64_t s = 0; s < nsrc; s++) {
     pos[3 * s + 0] = src[s].xpos;
     pos[3 * s + 1] = src[s].ypos;
     pos[3 * s + 2] = src[s].zpos;
-    memcpy(&spec[(size_t)s * ARGOT_NGRID_NU], src[s].photon_rate, sizeof(double) * ARGOT_NGRID_NU);
   }
+  static const double no_spec[ARGOT_NGRID_NU] = {};
+  const double* spec = nsrc > 0 ? src[0].photon_rate : no_spec;
   const TreeBox box{this_run->xmin, this_run->ymin, this_run->zmin,
                     this_run->xmax, this_run->ymax, this_run->zmax};
-  if (build_tree(pos.data(), spec.data(), (int)nsrc, ARGOT_NGRID_NU, box, this_run->theta_crit,
-                 &g.tree) != 0)
+  if (build_tree(pos.data(), spec, sizeof(struct radiation_src) / sizeof(double), (int)nsrc,
+                 ARGOT_NGRID_NU, box, this_run->theta_crit, &g.tree) != 0)
     return set_err(ARGOT_ERR_ARG);
   const HostTree& T = g.tree;
   const int nn = T.nnode();
   HIP_OK(g.link.reserve(nn));
   HIP_OK(g.cm.reserve(4 * (size_t)nn));
   HIP_OK(g.spec.reserve((size_t)nn * ARGOT_NGRID_NU));
-  HIP_OK(g.spec64.reserve((size_t)nn * ARGOT_NGRID_NU));
   HIP_OK(g.spec_scale.reserve(nn));
   HIP_OK(g.src_pos.reserve(T.src_pos.size()));
   HIP_OK(hipMemcpyAsync(g.link.p, T.link.data(), sizeof(NodeLink) * nn, hipMemcpyHostToDevice,
@@ -563,8 +566,6 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   HIP_OK(hipMemcpyAsync(g.cm.p, T.cm.data(), sizeof(float) * 4 * nn, hipMemcpyHostToDevice,
                         g.stream));
   HIP_OK(hipMemcpyAsync(g.spec.p, T.spec.data(), sizeof(float) * T.spec.size(),
-                        hipMemcpyHostToDevice, g.stream));
-  HIP_OK(hipMemcpyAsync(g.spec64.p, T.spec64.data(), sizeof(double) * T.spec64.size(),
                         hipMemcpyHostToDevice, g.stream));
   HIP_OK(hipMemcpyAsync(g.spec_scale.p, T.spec_scale.data(), sizeof(double) * nn,
                         hipMemcpyHostToDevice, g.stream));
@@ -582,7 +583,6 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   a.tree.link = g.link.p;
   a.tree.spec = g.spec.p;
   a.tree.spec_scale = g.spec_scale.p;
-  a.tree.spec64 = g.spec64.p;
   a.tree.src_pos = g.src_pos.p;
   a.rates = rate_tables();
   // Rate mode 1 needs a trained network; with no trusted cell every ray
@@ -996,7 +996,7 @@ void* argot_hip_tree_build(const float* pos, const double* spec, int nsrc, int n
                            const float* box6, float theta) {
   HostTree* t = new HostTree();
   const TreeBox b{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]};
-  if (build_tree(pos, spec, nsrc, ngrid, b, theta, t) != 0) {
+  if (build_tree(pos, spec, (size_t)std::max(ngrid, 0), nsrc, ngrid, b, theta, t) != 0) {
     delete t;
     return nullptr;
   }

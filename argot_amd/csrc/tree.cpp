@@ -25,9 +25,11 @@ struct Build {
   int ngrid;
   int nsrc;
   const float* pos;         // [3*nsrc]
-  const double* spec;       // [nsrc*ngrid]
+  const double* spec;       // source s's spectrum at spec + s*stride
+  size_t stride;
   std::vector<uint64_t> key;
   std::vector<int> order;   // sorted source order
+  std::vector<double> scratch;  // [ngrid]: the root's children sum (unused)
   float base_length;
   HostTree* out;
 };
@@ -76,6 +78,29 @@ int add_node(HostTree* t) {
   return id;
 }
 
+// Nodes below a node holding the sorted keys [first, first + n) at `level`:
+// the children plant() creates, counted the same way (so the arrays can be
+// reserved once).
+int count_nodes(const uint64_t* key, int first, int n, int level) {
+  level -= 3;
+  int total = 0;
+  for (int i = first; i < first + n;) {
+    const uint64_t oct = (key[i] >> level) & 7;
+    int j = i + 1;
+    while (j < first + n && ((key[j] >> level) & 7) == oct) j++;
+    total++;
+    if (j - i > 1 && level > 0) total += count_nodes(key, i, j - i, level);
+    i = j;
+  }
+  return total;
+}
+
+// Spectrum row of node u (u >= 0) or the scratch row (u < 0).  Rows are
+// addressed by index: add_node may move the array.
+double* spec_row(Build& b, int u) {
+  return u >= 0 ? &b.out->spec64[(size_t)u * b.ngrid] : b.scratch.data();
+}
+
 struct Sums {
   double L;
   double m[3];
@@ -83,9 +108,9 @@ struct Sums {
 
 // Pre-order: the caller has created node `self`; this creates and fills its
 // children (octant order), recursing into each child before creating the
-// next, and returns the subtree's luminosity, spectrum (into spec_out) and
-// L-weighted position sums.
-Sums plant(Build& b, int self, int first, int n, int level, double* spec_out) {
+// next, and returns the subtree's luminosity and L-weighted position sums;
+// the sum of the children's spectra goes to row `out` (zero on entry).
+Sums plant(Build& b, int self, int first, int n, int level, int out) {
   HostTree* t = b.out;
   const int ng = b.ngrid;
   level -= 3;
@@ -98,8 +123,6 @@ Sums plant(Build& b, int self, int first, int n, int level, double* spec_out) {
     prev = oct;
   }
   Sums tot{0.0, {0.0, 0.0, 0.0}};
-  for (int k = 0; k < ng; k++) spec_out[k] = 0.0;
-  std::vector<double> sub(ng);
   bool first_child = true;
   for (int c = 0; c < 8; c++) {
     if (!cnt[c]) continue;
@@ -112,27 +135,28 @@ Sums plant(Build& b, int self, int first, int n, int level, double* spec_out) {
     const float len = (float)(b.base_length * pow(2.0, -(63.0 - level) / 3.0));
     t->cm[4 * id + 3] = len * len;
     if (cnt[c] > 1 && level > 0) {
-      const Sums s = plant(b, id, beg[c], cnt[c], level, sub.data());
+      const Sums s = plant(b, id, beg[c], cnt[c], level, id);
       tot.L += s.L;
-      for (int k = 0; k < ng; k++) spec_out[k] += sub[k];
+      const double* sub = spec_row(b, id);
+      double* so = spec_row(b, out);
+      for (int k = 0; k < ng; k++) so[k] += sub[k];
       for (int d = 0; d < 3; d++) tot.m[d] += s.m[d];
-      memcpy(&t->spec64[(size_t)id * ng], sub.data(), sizeof(double) * ng);
       for (int d = 0; d < 3; d++) t->cm[4 * id + d] = (float)(s.m[d] / s.L);
     } else {
       double mt = 0.0, mp[3] = {0.0, 0.0, 0.0};
-      std::fill(sub.begin(), sub.end(), 0.0);
+      double* sub = spec_row(b, id);
       for (int j = beg[c]; j < beg[c] + cnt[c]; j++) {
         const int s = b.order[j];
-        const double* ps = b.spec + (size_t)s * ng;
+        const double* ps = b.spec + (size_t)s * b.stride;
         double m = 0.0;
         for (int k = 0; k < ng; k++) m += ps[k];
         for (int k = 0; k < ng; k++) sub[k] += ps[k];
         mt += m;
         for (int d = 0; d < 3; d++) mp[d] += m * b.pos[3 * s + d];
       }
-      memcpy(&t->spec64[(size_t)id * ng], sub.data(), sizeof(double) * ng);
       tot.L += mt;
-      for (int k = 0; k < ng; k++) spec_out[k] += sub[k];
+      double* so = spec_row(b, out);
+      for (int k = 0; k < ng; k++) so[k] += sub[k];
       for (int d = 0; d < 3; d++) tot.m[d] += mp[d];
       const double inv = 1.0 / mt;
       for (int d = 0; d < 3; d++) t->cm[4 * id + d] = (float)(mp[d] * inv);
@@ -145,9 +169,9 @@ Sums plant(Build& b, int self, int first, int n, int level, double* spec_out) {
 
 }  // namespace
 
-int build_tree(const float* pos, const double* spec, int nsrc, int ngrid, const TreeBox& box,
-               float theta_crit, HostTree* t) {
-  if (nsrc < 0 || ngrid <= 0 || ngrid > kNgridMax) return -1;
+int build_tree(const float* pos, const double* spec, size_t spec_stride, int nsrc, int ngrid,
+               const TreeBox& box, float theta_crit, HostTree* t) {
+  if (nsrc < 0 || ngrid <= 0 || ngrid > kNgridMax || spec_stride < (size_t)ngrid) return -1;
   t->ngrid = ngrid;
   t->nsrc = nsrc;
   t->theta_sq = theta_crit * theta_crit;
@@ -163,6 +187,8 @@ int build_tree(const float* pos, const double* spec, int nsrc, int ngrid, const 
   b.nsrc = nsrc;
   b.pos = pos;
   b.spec = spec;
+  b.stride = spec_stride;
+  b.scratch.assign(ngrid, 0.0);
   b.out = t;
   const float xl = box.xmax - box.xmin, yl = box.ymax - box.ymin, zl = box.zmax - box.zmin;
   const float byz = yl > zl ? yl : zl;
@@ -178,13 +204,17 @@ int build_tree(const float* pos, const double* spec, int nsrc, int ngrid, const 
     b.key[i] = interleave(ix, iy, iz);
   }
   if (nsrc > 0) sort_keys(b.order.data(), b.key.data(), 0, nsrc - 1);
+  const size_t nnode = 1 + (nsrc > 0 ? (size_t)count_nodes(b.key.data(), 0, nsrc, 63) : 0);
+  t->link.reserve(nnode);
+  t->cm.reserve(4 * nnode);
+  t->spec64.reserve(nnode * ngrid);
 
   // Root (construct_tree, :529-569).
   const int root = add_node(t);
   double xc = 0.0, yc = 0.0, zc = 0.0, lum = 0.0;
   double* all = &t->spec64[0];
   for (int s = 0; s < nsrc; s++) {
-    const double* ps = spec + (size_t)s * ngrid;
+    const double* ps = spec + (size_t)s * spec_stride;
     xc += pos[3 * s + 0] * ps[0];
     yc += pos[3 * s + 1] * ps[0];
     zc += pos[3 * s + 2] * ps[0];
@@ -200,10 +230,7 @@ int build_tree(const float* pos, const double* spec, int nsrc, int ngrid, const 
   t->cm[3] = b.base_length * b.base_length;
   t->link[root].n = nsrc;
   if (nsrc == 1) t->link[root].src = 0;
-  if (nsrc > 0) {
-    std::vector<double> tmp(ngrid);
-    plant(b, root, 0, nsrc, 63, tmp.data());
-  }
+  if (nsrc > 0) plant(b, root, 0, nsrc, 63, -1);
   t->link[root].skip = (int)t->link.size();
   // A node with n > 1 and no children (coincident keys at the deepest
   // level) keeps first = -1: opening it yields no ray, as in the reference.

@@ -30,9 +30,10 @@ struct HostRay {
   float pos[3];
 };
 
-// pos[3*i], spec[i*ngrid+k] of nsrc sources; returns 0 on success.
-int build_tree(const float* pos, const double* spec, int nsrc, int ngrid, const TreeBox& box,
-               float theta_crit, HostTree* t);
+// Positions pos[3*i] and spectra spec[i*spec_stride + k] (k < ngrid) of nsrc
+// sources; returns 0 on success.
+int build_tree(const float* pos, const double* spec, size_t spec_stride, int nsrc, int ngrid,
+               const TreeBox& box, float theta_crit, HostTree* t);
 
 // Rays for one target point (FP32 opening test as the kernel does it).
 uint64_t tree_rays_to(const HostTree& t, const float* target, std::vector<HostRay>* out);
