@@ -534,7 +534,18 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
     return set_err(ARGOT_ERR_ARG);   // the fused step covers one rank's whole box
   const int64_t nsrc = (int64_t)this_run->nsrc;
   if (nsrc < 0 || (nsrc > 0 && !src)) return set_err(ARGOT_ERR_ARG);
+  // Rate mode 1 needs a trained network.
+  if (g_rate_mode == 1 && !surrogate_ready()) return set_err(ARGOT_ERR_STATE);
   this_run->theta_crit = g_theta;
+
+  // Device work that does not need the tree goes first, so that it runs
+  // while the host builds the tree: per-cell FP32 densities, rates zeroed
+  // (zero_out_photoion_rate_cuda), counters reset.
+  HIP_OK(g.nf.reserve(g.ncell));
+  HIP_OK(hipEventRecord(g.ev0, g.stream));
+  HIP_OK(launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream));
+  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
+  HIP_OK(launch_reset_counters(g.ctr.p, g.stream));
 
   // Source tree on the host (construct_tree), uploaded in pre-order.  The
   // spectra are read where they lie in the radiation_src records.
@@ -573,8 +584,6 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
                         hipMemcpyHostToDevice, g.stream));
   const double t_built = now_ms();
 
-  HIP_OK(g.nf.reserve(g.ncell));
-
   FusedArgs a{};   // zero-initialised: the graph cache compares its bytes
   a.grid = make_grid(this_run);
   for (int d = 0; d < 3; d++) a.tile_bits[d] = g_tile_bits[d];
@@ -585,9 +594,8 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   a.tree.spec_scale = g.spec_scale.p;
   a.tree.src_pos = g.src_pos.p;
   a.rates = rate_tables();
-  // Rate mode 1 needs a trained network; with no trusted cell every ray
-  // would take the exact integral, so the plain kernel runs.
-  if (g_rate_mode == 1 && !surrogate_ready()) return set_err(ARGOT_ERR_STATE);
+  // With no trusted cell every ray would take the exact integral, so the
+  // plain kernel runs.
   const int surr = g_rate_mode == 1 && surrogate_any_trusted();
   if (surr) {
     std::vector<double> q;
@@ -608,15 +616,9 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   a.rec = capture ? g.rec.p : nullptr;
   a.rec_cap = capture ? g_capture_cap : 0;
   const int nblock = std::max(1, g.ncu * g_blocks_per_cu);
-  // Device work of the step: per-cell FP32 densities, rates zeroed
-  // (zero_out_photoion_rate_cuda), counters reset (small kernels), then the
-  // fused persistent kernel, timed alone between ev_mid and ev1.  The fused
+  // The fused persistent kernel, timed alone between ev_mid and ev1.  Its
   // launch is replayed from a HIP graph, captured again only when a launch
   // argument changes (the stream events stay outside the graph).
-  HIP_OK(hipEventRecord(g.ev0, g.stream));
-  HIP_OK(launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream));
-  HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
-  HIP_OK(launch_reset_counters(g.ctr.p, g.stream));
   auto enqueue = [&]() -> hipError_t {
     return launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream);
   };
