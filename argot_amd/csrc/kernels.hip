@@ -62,10 +62,44 @@ __global__ void add_rates_kernel(unsigned char* __restrict__ mesh, const float* 
   for (int k = 0; k < 6; k++) r[k] = r[k] + rate[6 * c + k];
 }
 
-// ray_segment records (56 B): start @16, end @28, tau @40.
-__global__ void walk_segments_kernel(GridDev g, const float4* __restrict__ nf,
-                                     unsigned char* __restrict__ seg, int64_t nseg, int kahan) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// ray_segment records (56 B): start @16, end @28, tau @40.  A block takes
+// 256 consecutive segments and hands them to its lanes in order of their
+// estimated crossing count (counting sort in LDS), so that the lanes of a
+// wavefront walk segments of similar length; each segment's result does not
+// depend on which lane walks it.
+constexpr int kWalkBlock = 256;
+__global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
+    GridDev g, const float4* __restrict__ nf, unsigned char* __restrict__ seg, int64_t nseg,
+    int kahan) {
+  __shared__ unsigned bucket_lds[64];
+  __shared__ unsigned perm_lds[kWalkBlock];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kWalkBlock;
+  if (t < 64) bucket_lds[t] = 0;
+  __syncthreads();
+  unsigned b = 0;
+  if (base + t < nseg) {
+    const float* r = reinterpret_cast<const float*>(seg + (base + t) * 56);
+    const float est = fabsf(r[7] - r[4]) / g.dx + fabsf(r[8] - r[5]) / g.dy +
+                      fabsf(r[9] - r[6]) / g.dz;
+    b = est < 504.0f ? (unsigned)est >> 3 : 63u;   // buckets of 8 crossings
+  }
+  const unsigned pos = atomicAdd(&bucket_lds[b], 1u);
+  __syncthreads();
+  if (t < 64) {   // exclusive prefix sum of the 64 bucket sizes (one wavefront)
+    const unsigned v = bucket_lds[t];
+    unsigned incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned u = __shfl_up(incl, d, 64);
+      if (t >= d) incl += u;
+    }
+    bucket_lds[t] = incl - v;
+  }
+  __syncthreads();
+  perm_lds[bucket_lds[b] + pos] = (unsigned)t;
+  __syncthreads();
+  const int64_t i = base + perm_lds[t];
   if (i >= nseg) return;
   float* rec = reinterpret_cast<float*>(seg + i * 56);
   const Vec3f s{rec[4], rec[5], rec[6]};
@@ -671,7 +705,7 @@ hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStr
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
                                 int kahan, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
-  walk_segments_kernel<<<nblocks(nseg, 256), 256, 0, s>>>(
+  walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
       g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
   return hipGetLastError();
 }
