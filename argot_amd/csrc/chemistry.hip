@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kBlock) chem_step_kernel(unsigned char* __rest
 // same as in chem_step_kernel; only the interleaving of cells differs.
 __global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* __restrict__ mesh,
                                                                  int64_t ncell, ChemParams p,
-                                                                 float dtime,
+                                                                 float dtime, int min_refill,
                                                                  ChemCounters* __restrict__ ctr) {
   const int lane = threadIdx.x & 63;
   unsigned int dchem = 0u, duene = 0u;
@@ -121,9 +121,12 @@ __global__ void __launch_bounds__(kBlock) chem_step_queue_kernel(unsigned char* 
   int64_t cell = -1;       // the lane's current cell, -1: needs one
   bool drained = false;    // the queue had no cell left for this lane
   for (;;) {
+    // Free lanes take new cells once min_refill of them are free, or when
+    // no lane has a cell left: the set-up of a cell then runs for several
+    // lanes at once instead of stalling the wavefront for one.
     const bool need = cell < 0 && !drained;
     const unsigned long long want = __ballot(need);
-    if (want) {
+    if (want && (__popcll(want) >= (unsigned)min_refill || __all(cell < 0))) {
       const int leader = __builtin_ctzll(want);
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(&ctr->next, (unsigned long long)__popcll(want));
@@ -217,12 +220,13 @@ unsigned int nblocks(int64_t ncell) { return (unsigned int)((ncell + kBlock - 1)
 }  // namespace
 
 hipError_t launch_chem_step(void* mesh, int64_t ncell, const ChemParams& p, float dtime,
-                            ChemCounters* ctr, int queue_blocks, hipStream_t s) {
+                            ChemCounters* ctr, int queue_blocks, int min_refill, hipStream_t s) {
   chem_reset_kernel<<<1, 64, 0, s>>>(ctr);
   if (ncell <= 0) return hipGetLastError();
   if (queue_blocks > 0) {
     const unsigned int nb = (unsigned int)std::min<int64_t>(queue_blocks, nblocks(ncell));
-    chem_step_queue_kernel<<<nb, kBlock, 0, s>>>((unsigned char*)mesh, ncell, p, dtime, ctr);
+    chem_step_queue_kernel<<<nb, kBlock, 0, s>>>((unsigned char*)mesh, ncell, p, dtime,
+                                                 std::max(min_refill, 1), ctr);
   } else {
     chem_step_kernel<<<nblocks(ncell), kBlock, 0, s>>>((unsigned char*)mesh, ncell, p, dtime, ctr);
   }

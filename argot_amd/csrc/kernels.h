@@ -44,7 +44,7 @@ hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int sur
 // queue_blocks > 0: the persistent cell-queue kernel on that many blocks;
 // 0: one cell per thread.
 hipError_t launch_chem_step(void* mesh, int64_t ncell, const ChemParams& p, float dtime,
-                            ChemCounters* ctr, int queue_blocks, hipStream_t s);
+                            ChemCounters* ctr, int queue_blocks, int min_refill, hipStream_t s);
 hipError_t launch_chem_dtime(const void* mesh, int64_t ncell, const ChemParams& p,
                              ChemCounters* ctr, hipStream_t s);
 hipError_t launch_chem_update(void* mesh, int64_t ncell, hipStream_t s);

@@ -54,7 +54,7 @@ int g_blocks_per_cu = 16;
 int g_tile_bits[3] = {2, 2, 2};        // 4x4x4 targets per wavefront
 int g_use_graph = 1;
 int g_iterations = 0;                  // 0: one point-source pass
-int g_chem_queue = 1;                  // persistent cell-queue chemistry kernel
+int g_chem_queue = 1;                  // cell-queue chemistry: free lanes refill at this count
 
 // Everything a captured step graph depends on.
 struct GraphKey {
@@ -707,7 +707,7 @@ static void chem_step(const run_param* r, float dtime, float* diff_chem, float* 
   HIP_OK(hipEventRecord(g.ev_c0, g.stream));
   const int qblocks = g_chem_queue ? 4 * g.ncu : 0;
   HIP_OK(launch_chem_step(g.mesh, g.ncell, chem_params(r), dtime, g.chem_ctr.p, qblocks,
-                          g.stream));
+                          g_chem_queue, g.stream));
   HIP_OK(hipEventRecord(g.ev_c1, g.stream));
   ChemCounters hc;
   HIP_OK(hipMemcpyAsync(&hc, g.chem_ctr.p, sizeof hc, hipMemcpyDeviceToHost, g.stream));
@@ -837,7 +837,7 @@ void argot_hip_chem_stats(struct argot_chem_stats* out) {
 
 void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }
 
-void argot_hip_set_chem_queue(int on) { g_chem_queue = on ? 1 : 0; }
+void argot_hip_set_chem_queue(int n) { g_chem_queue = n > 0 ? std::min(n, 64) : 0; }
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
 // streaming kernel; returns the best GB/s (read + write bytes / time).

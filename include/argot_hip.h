@@ -332,10 +332,12 @@ void argot_hip_chem_stats(struct argot_chem_stats *out);
  * change of chemistry or energy exceeds 1e-3, then update_chemistry_gpu;
  * the reference runs n = 5. */
 void argot_hip_set_iterations(int n);
-/* 1 (default): the chemistry kernel is persistent and takes cells from a
- * queue as its lanes finish, so cells with many sub-steps do not hold up a
- * wavefront; 0: one cell per thread.  Results are identical. */
-void argot_hip_set_chem_queue(int on);
+/* n >= 1: the chemistry kernel is persistent and takes cells from a queue
+ * as its lanes finish, so cells with many sub-steps do not hold up a
+ * wavefront; free lanes take new cells once n of them are free (or none is
+ * busy), n at most 64; 0: one cell per thread.  Results are identical for
+ * every n. */
+void argot_hip_set_chem_queue(int n);
 /* replaces merge_photoion_data, src/merge_photoion_data.cu:16-83:
  * mesh_dev[].prev_chem += host mesh[].prev_chem rates. */
 void merge_photoion_data(struct fluid_mesh *mesh,

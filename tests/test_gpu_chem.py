@@ -137,9 +137,11 @@ def test_iteration_loop_equals_its_calls(require_gpu):
     assert np.array_equal(got, want)
 
 
-def test_queue_kernel_equals_one_cell_per_thread(require_gpu):
-    """The persistent cell-queue kernel and the one-cell-per-thread kernel
-    give identical meshes and counts."""
+@pytest.mark.parametrize("refill", [1, 8, 64])
+def test_queue_kernel_equals_one_cell_per_thread(require_gpu, refill):
+    """The persistent cell-queue kernel (free lanes refilled once `refill`
+    of them are free) and the one-cell-per-thread kernel give identical
+    meshes and counts."""
     from argot_amd._lib import lib
     L = lib()
     st = stepped(n=16, seed=44)
@@ -151,7 +153,7 @@ def test_queue_kernel_equals_one_cell_per_thread(require_gpu):
         m0 = st.mesh_bytes()
         np.copyto(np.frombuffer(st.mesh, np.uint8), ref)
         st.send_mesh()
-        L.argot_hip_set_chem_queue(1)
+        L.argot_hip_set_chem_queue(refill)
         d1 = st.chemistry(np.float32(3e-3))
         c1 = st.chem_stats()
         m1 = st.mesh_bytes()
