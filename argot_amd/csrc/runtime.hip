@@ -312,6 +312,16 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   (void)mesh;
   t_err = ARGOT_OK;
   if (!cuda_mem || !this_cuda || !this_run) return set_err(ARGOT_ERR_ARG);
+  // Mesh sizes the kernels' index arithmetic covers: ix*ny + iy in 24 bits
+  // (24-bit multiply-adds form the cell index), the cell count in an int
+  // (cuda_param.nmesh_per_dev) and so in the 32-bit linear cell index.
+  {
+    const int64_t nx = this_run->nmesh_x_local, ny = this_run->nmesh_y_local,
+                  nz = this_run->nmesh_z_local;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx * ny >= (int64_t)1 << 24 ||
+        nx * ny * nz > (int64_t)INT32_MAX)
+      return set_err(ARGOT_ERR_ARG);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err(ARGOT_ERR_HIP);
   HIP_OK(hipSetDevice(0));
@@ -321,7 +331,6 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   g.nx = this_run->nmesh_x_local;
   g.ny = this_run->nmesh_y_local;
   g.nz = this_run->nmesh_z_local;
-  if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0) return set_err(ARGOT_ERR_ARG);
   g.ncell = (int64_t)g.nx * g.ny * g.nz;
   memset(this_cuda, 0, sizeof(*this_cuda));
   this_cuda->num_cuda_dev = 1;

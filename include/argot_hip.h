@@ -194,7 +194,9 @@ void setup_cross_section(struct cross_section *csect, struct freq_param *freq);
 /* ------------------------------------------------------------------ */
 
 /* replaces init_gpu, src/init_gpu.cu:14-145: one device (num_cuda_dev=1),
- * device-resident mesh (AoS, as the reference) and run_param, stream. */
+ * device-resident mesh (AoS, as the reference) and run_param, stream.
+ * ARGOT_ERR_ARG for a local mesh with a non-positive size, nx*ny >= 2^24 or
+ * more than 2^31-1 cells. */
 void init_gpu(struct fluid_mesh *mesh, struct cuda_mem_space *cuda_mem,
               struct cuda_param *this_cuda, struct run_param *this_run);
 /* replaces send_mesh_data, src/send_mesh_data.cu:8-29 */

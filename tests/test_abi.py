@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from argot_amd import abi
 from argot_amd._lib import EXPORTS, LIB_PATH, lib
@@ -56,6 +57,19 @@ def test_gpu_entry_points_fail_cleanly_without_init():
     L.zero_out_photoion_rate_cuda(C.byref(cm), C.byref(cp), C.byref(run))
     assert L.argot_hip_last_error() == 3
     assert L.argot_hip_error_string(3).decode().startswith("init_gpu")
+
+
+@pytest.mark.parametrize("shape", [(0, 8, 8), (8, -1, 8), (4096, 4096, 1), (2048, 2048, 2048)])
+def test_init_gpu_rejects_meshes_outside_the_index_range(shape):
+    """Sizes are checked before any device call: empty meshes, nx*ny of 2^24
+    or more (24-bit cell-index arithmetic) and more than 2^31-1 cells."""
+    L = lib()
+    run = abi.RunParam()
+    run.nmesh_x_local, run.nmesh_y_local, run.nmesh_z_local = shape
+    cm = abi.CudaMemSpace()
+    cp = abi.CudaParam()
+    L.init_gpu(None, C.byref(cm), C.byref(cp), C.byref(run))
+    assert L.argot_hip_last_error() == 2    # ARGOT_ERR_ARG
 
 
 def test_host_tree_handles_tiny_inputs():
