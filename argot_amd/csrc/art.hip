@@ -364,61 +364,44 @@ __global__ void art_fill_kernel(float4* __restrict__ p, int64_t n) {
   if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// The slices of the batch's z-dominant directions into the other directions'
-// layout (acc -> out), one 32x32 (z, y) tile of one x plane per block through
-// LDS so that both the reads (y fastest) and the writes (z fastest) are
-// contiguous runs.
-// Grid: (ceil(ny/32) ceil(nz/32), nx, ndir); blocks of other directions
-// return at once.
-__global__ __launch_bounds__(256) void art_relayout_kernel(const float4* __restrict__ acc,
-                                                           float4* __restrict__ out,
-                                                           const ArtDir* __restrict__ dirs,
-                                                           int nx, int ny, int nz) {
-  const int d = blockIdx.z;
-  if (dirs[d].base > 1) return;
-  __shared__ float4 tile[32][33];
-  const int ntz = (nz + 31) / 32;
-  const int y0 = (int)(blockIdx.x / ntz) * 32, z0 = (int)(blockIdx.x % ntz) * 32;
-  const int ix = blockIdx.y;
-  const int64_t ncell = (int64_t)nx * ny * nz;
-  const float4* src = acc + (int64_t)d * ncell;
-  float4* dst = out + (int64_t)d * ncell;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-  for (int r = ty; r < 32; r += 8) {
-    const int iz = z0 + r, iy = y0 + tx;
-    if (iz < nz && iy < ny) tile[r][tx] = src[art_index(true, nx, ny, nz, ix, iy, iz)];
-  }
-  __syncthreads();
-  for (int r = ty; r < 32; r += 8) {
-    const int iy = y0 + r, iz = z0 + tx;
-    if (iz < nz && iy < ny) dst[art_index(false, nx, ny, nz, ix, iy, iz)] = tile[tx][r];
-  }
-}
-
 // ----------------------------------------------------------------------
 // calc_GH_tot for the batch's directions in order: gh[k][c] (FP32, 6
 // outputs GHI, HHI, GHeI, HHeI, GHeII, HHeII); optionally Kahan-compensated
 // across directions.
+// One block per 32 x 32 (y, z) tile of one x plane, one thread per cell.
+// The slices of x- and y-dominant directions are read in their own layout
+// (z fastest); those of z-dominant directions (y fastest) go through an LDS
+// tile, read along y and used along z, so that every read is a contiguous
+// run.  Each cell's sums take the directions in batch order.
+// Grid: (ceil(ny/32) ceil(nz/32), nx), blocks of kGhTile^2 threads.
+constexpr int kGhTile = 32;
 template <bool KAHAN>
-__global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __restrict__ accm,
-                              const ArtDir* __restrict__ dirs, int ndir, int nx, int ny, int nz,
-                              const float4* __restrict__ med,
-                              float f0, float f1, float f2, float f3, float f4, float f5,
-                              float* __restrict__ gh, float* __restrict__ comp) {
+__global__ __launch_bounds__(kGhTile * kGhTile) void art_gh_kernel(
+    const float4* __restrict__ acc, const ArtDir* __restrict__ dirs, int ndir, int nx, int ny,
+    int nz, const float4* __restrict__ med, float f0, float f1, float f2, float f3, float f4,
+    float f5, float* __restrict__ gh, float* __restrict__ comp) {
+  __shared__ float4 tile[kGhTile][kGhTile + 1];
   const int64_t ncell = (int64_t)nx * ny * nz;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  // this cell's records: the layout of x- and y-dominant directions (the
-  // z-dominant ones were relaid into it)
-  const int cix = (int)(c / ((int64_t)ny * nz));
-  const int ciy = (int)((c / nz) % ny);
-  const int64_t cs = art_index(false, nx, ny, nz, cix, ciy, (int)(c % nz));
-  const float4 ka = med[2 * cs], so = med[2 * cs + 1];
+  const int ntz = (nz + kGhTile - 1) / kGhTile;
+  const int y0 = (int)(blockIdx.x / ntz) * kGhTile, z0 = (int)(blockIdx.x % ntz) * kGhTile;
+  const int ix = blockIdx.y;
+  const int tz = threadIdx.x % kGhTile, ty = threadIdx.x / kGhTile;
+  const int iy = y0 + ty, iz = z0 + tz;
+  const bool valid = iy < ny && iz < nz;
+  // this cell's record in the layout of x- and y-dominant directions
+  const int64_t cs = valid ? art_index(false, nx, ny, nz, ix, iy, iz) : 0;
+  const int64_t c = ((int64_t)ix * ny + iy) * nz + iz;
+  // the record this thread loads into the tile for z-dominant directions:
+  // cell (y0 + tz, z0 + ty), read along y
+  const bool tvalid = y0 + tz < ny && z0 + ty < nz;
+  const int64_t ct = tvalid ? art_index(true, nx, ny, nz, ix, y0 + tz, z0 + ty) : 0;
+  const float4 ka = valid ? med[2 * cs] : make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  const float4 so = valid ? med[2 * cs + 1] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float s[6], e[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    s[k] = gh[k * ncell + c];
-    e[k] = KAHAN ? comp[k * ncell + c] : 0.0f;
+    s[k] = valid ? gh[k * ncell + c] : 0.0f;
+    e[k] = KAHAN && valid ? comp[k * ncell + c] : 0.0f;
   }
   const double f[6] = {f0, f1, f2, f3, f4, f5};
   // Per-cell reciprocals: the float quotients I/kappa are exact through
@@ -437,9 +420,16 @@ __global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __re
   sh1 += (double)so.z * kRhHeI3;
   const double sg2 = so.z, sh2 = so.z;
   for (int d = 0; d < ndir; d++) {
-    // z-dominant directions' slices were relaid into accm
-    const float4* src = dirs[d].base <= 1 ? accm : acc;
-    const float4 a = src[(int64_t)d * ncell + cs];
+    float4 a;
+    if (dirs[d].base <= 1) {   // the same for the whole block
+      __syncthreads();         // the previous tile has been read
+      if (tvalid) tile[ty][tz] = acc[(int64_t)d * ncell + ct];
+      __syncthreads();
+      a = tile[tz][ty];
+    } else {
+      a = acc[(int64_t)d * ncell + cs];
+    }
+    if (!valid) continue;
     const double rl = 1.0 / (double)a.x;
     const double q1 = (double)div_by(a.y, rkx);   // I_1 / kappa_1 (float, as the reference)
     const double q2 = (double)div_by(a.z, rky);
@@ -477,6 +467,7 @@ __global__ void art_gh_kernel(const float4* __restrict__ acc, const float4* __re
       }
     }
   }
+  if (!valid) return;
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     gh[k * ncell + c] = s[k];
@@ -516,7 +507,7 @@ struct ArtCtx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ArtDir* d_dirs = nullptr;
   float4 *med = nullptr, *acc = nullptr;   // med: [2*ncell] absorption, source
-  float4 *medT = nullptr, *accm = nullptr; // y-fastest medium; acc slices in mesh order
+  float4* medT = nullptr;   // y-fastest medium
   float *gh = nullptr, *comp = nullptr;
   unsigned long long* ncross = nullptr;
   int64_t ncell = 0;
@@ -716,11 +707,11 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
     return argot_hip_internal_set_error(ARGOT_ERR_ARG);   // ray groups need even sizes
   const int64_t ncell = (int64_t)nx * ny * nz;
   const int nang = 12 * g_nside * g_nside;
-  // Each direction of a batch owns two float4 slices (trace layout and
-  // relaid); more directions per batch keep more rays in flight.
+  // Each direction of a batch owns one float4 slice over the mesh; more
+  // directions per batch keep more rays in flight.
   int batch = g_batch;
   if (batch <= 0) {
-    const int64_t fit = ((int64_t)8 << 30) / (32 * ncell);
+    const int64_t fit = ((int64_t)8 << 30) / (16 * ncell);
     batch = (int)std::max<int64_t>(1, std::min<int64_t>(128, fit));
   }
   batch = std::min(batch, nang);
@@ -729,11 +720,11 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
     for (auto& e : a_.ev) A_OK(hipEventCreate(&e));
   }
   for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
-                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT, (void*)a_.accm})
+                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT})
     if (p) A_OK(hipFree(p));
   a_.d_dirs = nullptr;
   a_.med = a_.acc = nullptr;
-  a_.medT = a_.accm = nullptr;
+  a_.medT = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
@@ -741,7 +732,6 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
   A_OK(hipMalloc(&a_.med, sizeof(float4) * 2 * ncell));
   A_OK(hipMalloc(&a_.acc, sizeof(float4) * ncell * batch));
   A_OK(hipMalloc(&a_.medT, sizeof(float4) * 2 * ncell));
-  A_OK(hipMalloc(&a_.accm, sizeof(float4) * ncell * batch));
   A_OK(hipMalloc(&a_.gh, sizeof(float) * 6 * ncell));
   A_OK(hipMalloc(&a_.comp, sizeof(float) * 6 * ncell));
   A_OK(hipMalloc(&a_.ncross, sizeof(unsigned long long)));
@@ -852,22 +842,17 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
         art_trace_kernel<false><<<(unsigned)((nthread + 255) / 256), 256, 0, st>>>(ta);
     }
     A_OK(hipEventRecord(a_.ev[2], st));
-    {
-      const dim3 grid((unsigned)(((g.ny + 31) / 32) * ((g.nz + 31) / 32)), (unsigned)g.nx,
-                      (unsigned)nd);
-      art_relayout_kernel<<<grid, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, g.nx, g.ny,
-                                                g.nz);
-    }
     const float* f = a_.fact;
+    const dim3 ghgrid((unsigned)(((g.ny + kGhTile - 1) / kGhTile) * ((g.nz + kGhTile - 1) / kGhTile)),
+                      (unsigned)g.nx);
     if (g_kahan)
-      art_gh_kernel<true><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, g.nx, g.ny,
-                                              g.nz, a_.med,
-                                              f[0], f[1], f[2], f[3], f[4], f[5], a_.gh,
-                                              a_.comp);
+      art_gh_kernel<true><<<ghgrid, kGhTile * kGhTile, 0, st>>>(
+          a_.acc, a_.d_dirs + d0, nd, g.nx, g.ny, g.nz, a_.med, f[0], f[1], f[2], f[3], f[4],
+          f[5], a_.gh, a_.comp);
     else
-      art_gh_kernel<false><<<nb, 256, 0, st>>>(a_.acc, a_.accm, a_.d_dirs + d0, nd, g.nx,
-                                               g.ny, g.nz, a_.med, f[0], f[1], f[2], f[3], f[4], f[5],
-                                               a_.gh, a_.comp);
+      art_gh_kernel<false><<<ghgrid, kGhTile * kGhTile, 0, st>>>(
+          a_.acc, a_.d_dirs + d0, nd, g.nx, g.ny, g.nz, a_.med, f[0], f[1], f[2], f[3], f[4],
+          f[5], a_.gh, a_.comp);
     A_OK(hipGetLastError());
     A_OK(hipEventRecord(a_.ev[3], st));
     A_OK(hipEventSynchronize(a_.ev[3]));
@@ -893,11 +878,11 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
 void free_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cuda) {
   (void)this_cuda;
   for (void* p : {(void*)a_.d_dirs, (void*)a_.med, (void*)a_.acc, (void*)a_.gh,
-                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT, (void*)a_.accm})
+                  (void*)a_.comp, (void*)a_.ncross, (void*)a_.medT})
     if (p) (void)hipFree(p);
   a_.d_dirs = nullptr;
   a_.med = a_.acc = nullptr;
-  a_.medT = a_.accm = nullptr;
+  a_.medT = nullptr;
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
