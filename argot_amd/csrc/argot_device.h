@@ -255,13 +255,25 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float term) {
   s = t;
 }
 
+// Fields of cell idx.  NARROW: the byte offset idx*16 fits in 32 bits (the
+// mesh has fewer than 2^28 cells), so the load adds it to the scalar base
+// without a 64-bit address computation.
+constexpr int64_t kNarrowCells = (int64_t)1 << 28;
+template <bool NARROW>
+__device__ __forceinline__ float4 cell_fields(const float4* __restrict__ nf, uint32_t idx) {
+  if constexpr (NARROW)
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nf) + (idx << 4));
+  else
+    return nf[idx];
+}
+
 // The whole walk of one segment (calc_optical_depth.c:76-227).  nf holds the
 // per-cell FP32 densities (nHI, nHeI, nHeII) formed as the reference forms
 // them.  The next cell's densities are fetched before the current chord is
 // summed, so one gather is always in flight.  KAHAN selects compensated
 // sums; HASH records the cell sequence (FNV-1a over the linear cell
 // indices, the end cell last).
-template <bool KAHAN, bool HASH>
+template <bool KAHAN, bool HASH, bool NARROW>
 __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__ nf, Vec3f s,
                                     Vec3f e, WalkOut* out) {
   Dda d;
@@ -271,13 +283,13 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
   bool act = dda_active(d);
-  float4 n = nf[act ? d.idx : 0u];
+  float4 n = cell_fields<NARROW>(nf, act ? d.idx : 0u);
   while (act) {
     if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
     nc++;
     const float rmin = dda_step(&d);
     const bool nact = dda_active(d);
-    const float4 nn = nf[nact ? d.idx : 0u];
+    const float4 nn = cell_fields<NARROW>(nf, nact ? d.idx : 0u);
     if (KAHAN) {
       kahan_add(t0, c0, rmin * n.x);
       kahan_add(t1, c1, rmin * n.y);
@@ -292,7 +304,7 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   }
   const float depth = dda_depth(g, d, e);
   const uint32_t ie = d.idxe;
-  n = nf[ie];
+  n = cell_fields<NARROW>(nf, ie);
   if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
   nc++;
   if (KAHAN) {

@@ -24,8 +24,10 @@ hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fac
                               hipStream_t s);
 hipError_t launch_zero_rates(void* mesh, int64_t ncell, hipStream_t s);
 hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s);
+// wide: address the cell fields in 64 bits even when the mesh has fewer
+// than 2^28 cells (meshes with more always are).
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
-                                int kahan, hipStream_t s);
+                                int kahan, int wide, hipStream_t s);
 hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* mesh, const void* rays, int64_t nray, double* per_ray,
                             int64_t* target, hipStream_t s);
 hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
@@ -37,8 +39,8 @@ hipError_t launch_smooth_rates(void* mesh, int nx, int ny, int nz, float* tmp_in
                                hipStream_t s);
 hipError_t launch_copy_probe(const float4* src, float4* dst, int64_t n, int nblock,
                              hipStream_t s);
-hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,
-                             hipStream_t s);
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int wide,
+                             int nblock, hipStream_t s);
 
 // Chemistry (chemistry.hip).  Each launch first resets *ctr on the stream.
 // queue_blocks > 0: the persistent cell-queue kernel on that many blocks;

@@ -68,6 +68,7 @@ __global__ void add_rates_kernel(unsigned char* __restrict__ mesh, const float* 
 // wavefront walk segments of similar length; each segment's result does not
 // depend on which lane walks it.
 constexpr int kWalkBlock = 256;
+template <bool NARROW>
 __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
     GridDev g, const float4* __restrict__ nf, unsigned char* __restrict__ seg, int64_t nseg,
     int kahan) {
@@ -106,9 +107,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
   const Vec3f e{rec[7], rec[8], rec[9]};
   WalkOut w;
   if (kahan)
-    walk_segment<true, false>(g, nf, s, e, &w);
+    walk_segment<true, false, NARROW>(g, nf, s, e, &w);
   else
-    walk_segment<false, false>(g, nf, s, e, &w);
+    walk_segment<false, false, NARROW>(g, nf, s, e, &w);
   rec[10] = w.tau[0];
   rec[11] = w.tau[1];
   rec[12] = w.tau[2];
@@ -424,7 +425,7 @@ __device__ inline void surr_rates(const RayRate& rr, double q, const float ya[kN
   res[5] = p2 * m[5] * kHnuL;
 }
 
-template <bool KAHAN, bool CAPTURE, bool SURR>
+template <bool KAHAN, bool CAPTURE, bool SURR, bool NARROW>
 __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArgs a) {
   const GridDev& g = a.grid;
   const TreeDev& tr = a.tree;
@@ -502,7 +503,7 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
                       tr.src_pos[4 * ln.src + 2]};
         }
         segment_start(g, src, tgt, &st);
-        walk_segment<KAHAN, CAPTURE>(g, a.nf, st, tgt, &w);
+        walk_segment<KAHAN, CAPTURE, NARROW>(g, a.nf, st, tgt, &w);
         const float4 f = cell_lds[lane];
         ray_rate_setup(g, rt, w.tau, src, tgt, f.x, f.y, f.z, f.w, &rr);
       }
@@ -703,10 +704,14 @@ hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStr
 }
 
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
-                                int kahan, hipStream_t s) {
+                                int kahan, int wide, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
-  walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-      g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
+  if (!wide && (int64_t)g.nx * g.ny * g.nz < kNarrowCells)
+    walk_segments_kernel<true><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
+  else
+    walk_segments_kernel<false><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
   return hipGetLastError();
 }
 
@@ -732,27 +737,35 @@ hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <bool SURR>
+template <bool SURR, bool NARROW>
 static void launch_fused(const FusedArgs& a, int kahan, int capture, int nblock, hipStream_t s) {
   if (capture) {
     if (kahan)
-      fused_step_kernel<true, true, SURR><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, true, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, true, SURR><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, true, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
   } else {
     if (kahan)
-      fused_step_kernel<true, false, SURR><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, false, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, false, SURR><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, false, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
   }
 }
 
-hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int nblock,
-                             hipStream_t s) {
-  if (surr)
-    launch_fused<true>(a, kahan, capture, nblock, s);
-  else
-    launch_fused<false>(a, kahan, capture, nblock, s);
+hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int wide,
+                             int nblock, hipStream_t s) {
+  const bool narrow = !wide && (int64_t)a.grid.nx * a.grid.ny * a.grid.nz < kNarrowCells;
+  if (surr) {
+    if (narrow)
+      launch_fused<true, true>(a, kahan, capture, nblock, s);
+    else
+      launch_fused<true, false>(a, kahan, capture, nblock, s);
+  } else {
+    if (narrow)
+      launch_fused<false, true>(a, kahan, capture, nblock, s);
+    else
+      launch_fused<false, false>(a, kahan, capture, nblock, s);
+  }
   return hipGetLastError();
 }
 

@@ -54,12 +54,13 @@ int g_blocks_per_cu = 16;
 int g_tile_bits[3] = {2, 2, 2};        // 4x4x4 targets per wavefront
 int g_use_graph = 1;
 int g_iterations = 0;                  // 0: one point-source pass
+int g_wide_index = 0;                  // 1: 64-bit cell-field addresses on any mesh
 int g_chem_queue = 1;                  // cell-queue chemistry: free lanes refill at this count
 
 // Everything a captured step graph depends on.
 struct GraphKey {
   FusedArgs args;
-  int nblock, kahan, surr;
+  int nblock, kahan, surr, wide;
   int64_t ncell;
   double helium_fact;
   const float4* nf;
@@ -431,7 +432,7 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
   HIP_OK(hipMemcpyAsync(g.seg.p, seg, bytes, hipMemcpyHostToDevice, g.stream));
   const GridDev gd = make_grid(this_run);
   HIP_OK(hipEventRecord(g.ev_c0, g.stream));
-  HIP_OK(launch_walk_segments(gd, g.nf.p, g.seg.p, nseg, g_tau_mode, g.stream));
+  HIP_OK(launch_walk_segments(gd, g.nf.p, g.seg.p, nseg, g_tau_mode, g_wide_index, g.stream));
   HIP_OK(hipEventRecord(g.ev_c1, g.stream));
   HIP_OK(hipMemcpyAsync(seg, g.seg.p, bytes, hipMemcpyDeviceToHost, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
@@ -629,7 +630,7 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   // launch is replayed from a HIP graph, captured again only when a launch
   // argument changes (the stream events stay outside the graph).
   auto enqueue = [&]() -> hipError_t {
-    return launch_fused_step(a, g_tau_mode, capture, surr, nblock, g.stream);
+    return launch_fused_step(a, g_tau_mode, capture, surr, g_wide_index, nblock, g.stream);
   };
   const bool use_graph = g_use_graph && !capture;
   if (use_graph) {
@@ -638,6 +639,7 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
     key.nblock = nblock;
     key.kahan = g_tau_mode;
     key.surr = surr;
+    key.wide = g_wide_index;
     key.ncell = g.ncell;
     key.helium_fact = g_helium_fact;
     key.nf = g.nf.p;
@@ -846,6 +848,7 @@ void argot_hip_chem_stats(struct argot_chem_stats* out) {
 
 void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }
 
+void argot_hip_set_wide_index(int on) { g_wide_index = on ? 1 : 0; }
 void argot_hip_set_chem_queue(int n) { g_chem_queue = n > 0 ? std::min(n, 64) : 0; }
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain

@@ -340,6 +340,11 @@ void argot_hip_set_iterations(int n);
  * busy), n at most 64; 0: one cell per thread.  Results are identical for
  * every n. */
 void argot_hip_set_chem_queue(int n);
+/* 1: the walks address the per-cell fields with 64-bit offsets on any mesh;
+ * 0 (default): with 32-bit byte offsets when the mesh has fewer than 2^28
+ * cells, 64-bit ones otherwise.  Results are identical (a test control for
+ * the large-mesh path). */
+void argot_hip_set_wide_index(int on);
 /* replaces merge_photoion_data, src/merge_photoion_data.cu:16-83:
  * mesh_dev[].prev_chem += host mesh[].prev_chem rates. */
 void merge_photoion_data(struct fluid_mesh *mesh,
