@@ -5,8 +5,10 @@ the mesh and of four more at source cells is compared with the oracle
 (relative 1e-5, the same inf/NaN pattern); the whole mesh is checked for
 what holds at any size: a rerun is bit-identical, finite rates are
 non-negative, and non-finite ones (rays whose source sits at the target
-centre) are confined to a few cells.  The chemistry of the whole clumped256
-mesh is checked on sampled cells against the oracle."""
+centre) are confined to a few cells.  On clumped256, rays captured from the
+step are checked one by one against the oracle's (optical-depth bits,
+crossings, cell sequence), and the chemistry of the whole mesh is checked on
+sampled cells."""
 import numpy as np
 import pytest
 
@@ -99,3 +101,40 @@ def test_full_size_chemistry(require_gpu):
     np.testing.assert_allclose(a, b, rtol=2e-5, err_msg="uene")
     print(f"{exact.mean():.4f} of {len(cells)} sampled cells bit-exact", flush=True)
     assert exact.mean() >= 0.9
+
+
+def test_full_size_rays_match_oracle(require_gpu):
+    """clumped256: the first 2 Mi rays the fused step emits (whole prefixes
+    of their targets' ray lists, by seq) against the oracle ray for ray, for
+    2000 sampled targets: kind, source, source position, optical-depth bits,
+    crossings and cell-sequence hash."""
+    w = ic.clumped256()
+    step = R.PointSourceStep(w)
+    rec, st = step.capture(2 << 20)
+    assert len(rec) == 2 << 20 < st["nray"]
+    P = oracle_problem(w)
+    rng = np.random.default_rng(9)
+    targets = rng.choice(np.unique(rec["target"]), 2000, replace=False)
+    starts = np.searchsorted(rec["target"], targets, side="left")
+    ends = np.searchsorted(rec["target"], targets, side="right")
+    nray = 0
+    for t, a, b in zip(targets, starts, ends):
+        r = rec[a:b]
+        orc = P.ray_records(int(t), int(t) + 1)
+        kind, ids, pos = P.rays_to(*[int(v) for v in
+                                     (t // (w.n * w.n), (t // w.n) % w.n, t % w.n)])
+        seq = r["seq"]
+        assert np.array_equal(seq, np.arange(len(r))), f"target {t}: not a prefix"
+        assert len(r) <= len(orc["kind"])
+        assert np.array_equal(r["kind"], orc["kind"][seq]), f"target {t} kinds"
+        src = r["kind"] == 1
+        assert np.array_equal(r["id"][src], orc["id"][seq][src]), f"target {t} sources"
+        assert np.array_equal(r["src"].view(np.uint32), pos[seq].view(np.uint32)), \
+            f"target {t} source positions"
+        assert np.array_equal(r["tau"].view(np.uint32), orc["tau"][seq].view(np.uint32)), \
+            f"target {t} optical depths"
+        assert np.array_equal(r["ncross"], orc["ncross"][seq]), f"target {t} crossings"
+        assert np.array_equal(r["hash"], orc["hash"][seq]), f"target {t} cell sequences"
+        nray += len(r)
+    print(f"{nray} rays of {len(targets)} targets equal the oracle's", flush=True)
+    assert nray > 2000
