@@ -19,6 +19,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "argot_hip.h"
@@ -51,7 +52,19 @@ struct File {
   }
   bool read(void* p, size_t n) { return n == 0 || fread(p, 1, n, f) == n; }
   bool write(const void* p, size_t n) { return n == 0 || fwrite(p, 1, n, f) == n; }
+  // bytes from the current position to the end, or -1
+  int64_t remaining() {
+    const long here = ftell(f);
+    if (here < 0 || fseek(f, 0, SEEK_END) != 0) return -1;
+    const long end = ftell(f);
+    if (end < 0 || fseek(f, here, SEEK_SET) != 0) return -1;
+    return (int64_t)end - here;
+  }
 };
+
+// Cell records move through a fixed buffer of this many records, so that no
+// allocation scales with the mesh.
+constexpr int64_t kIoChunk = 65536;
 
 int64_t local_cells(const run_param* r) {
   if (r->nmesh_x_local <= 0 || r->nmesh_y_local <= 0 || r->nmesh_z_local <= 0) return -1;
@@ -95,12 +108,15 @@ void input_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char
   run_param h;
   if (!in.read(&h, sizeof h)) return fail(ARGOT_ERR_IO);
   if (!header_fits(h, this_run)) return fail(ARGOT_ERR_ARG);
-  std::vector<mesh_io> rec((size_t)n);
-  if (!in.read(rec.data(), sizeof(mesh_io) * rec.size())) return fail(ARGOT_ERR_IO);
-  memcpy(this_run, &h, sizeof h);
+  // a short file is refused before any cell is touched
+  if (in.remaining() < n * (int64_t)sizeof(mesh_io)) return fail(ARGOT_ERR_IO);
+  std::vector<mesh_io> rec((size_t)std::min(n, kIoChunk));
   for (int64_t i = 0; i < n; i++) {
+    const int64_t j = i % kIoChunk;
+    if (j == 0 && !in.read(rec.data(), sizeof(mesh_io) * (size_t)std::min(n - i, kIoChunk)))
+      return fail(ARGOT_ERR_IO);
     fluid_mesh& m = mesh[i];
-    const mesh_io& t = rec[(size_t)i];
+    const mesh_io& t = rec[(size_t)j];
     m.dens = t.dens;
     m.eneg = t.eneg;
     m.momx = t.momx;
@@ -112,6 +128,7 @@ void input_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char
     m.prev_chem = t.chem;
     m.prev_uene = t.uene;
   }
+  memcpy(this_run, &h, sizeof h);
 }
 
 void input_mesh(struct fluid_mesh* mesh, struct run_param* this_run, char* prefix) {
@@ -163,10 +180,14 @@ void output_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, cha
   if (!mesh || !this_run || !filename) return fail(ARGOT_ERR_ARG);
   const int64_t n = local_cells(this_run);
   if (n < 0) return fail(ARGOT_ERR_ARG);
-  std::vector<mesh_io> rec((size_t)n);
+  File out(filename, "wb");
+  if (!out.f) return fail(ARGOT_ERR_IO);
+  if (!out.write(this_run, sizeof(run_param))) return fail(ARGOT_ERR_IO);
+  std::vector<mesh_io> rec((size_t)std::min(n, kIoChunk));
   for (int64_t i = 0; i < n; i++) {
+    const int64_t j = i % kIoChunk;
     const fluid_mesh& m = mesh[i];
-    mesh_io& t = rec[(size_t)i];
+    mesh_io& t = rec[(size_t)j];
     memset(&t, 0, sizeof t);
     t.dens = m.dens;
     t.eneg = m.eneg;
@@ -176,12 +197,10 @@ void output_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, cha
     t.uene = m.uene;
     t.pot = m.pot;
     t.chem = m.chem;
+    if ((j == kIoChunk - 1 || i == n - 1) &&
+        !out.write(rec.data(), sizeof(mesh_io) * (size_t)(j + 1)))
+      return fail(ARGOT_ERR_IO);
   }
-  File out(filename, "wb");
-  if (!out.f) return fail(ARGOT_ERR_IO);
-  if (!out.write(this_run, sizeof(run_param)) ||
-      !out.write(rec.data(), sizeof(mesh_io) * rec.size()))
-    return fail(ARGOT_ERR_IO);
 }
 
 void output_mesh(struct fluid_mesh* mesh, struct run_param* this_run, char* prefix) {
