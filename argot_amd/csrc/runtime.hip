@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
+#include <new>
 #include <vector>
 
 #include "argot_hip.h"
@@ -261,7 +263,7 @@ const char* argot_hip_error_string(int code) {
     case ARGOT_ERR_HIP: return "HIP runtime call failed";
     case ARGOT_ERR_ARG: return "bad argument or size";
     case ARGOT_ERR_STATE: return "init_gpu / send_mesh_data not called, or no trained surrogate";
-    case ARGOT_ERR_NOMEM: return "device allocation failed";
+    case ARGOT_ERR_NOMEM: return "device or host allocation failed";
     case ARGOT_ERR_TOO_MANY_SEGMENTS: return "ray crosses too many domains";
     case ARGOT_ERR_IO: return "file could not be opened or is short";
     default: return "unknown";
@@ -441,9 +443,9 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
   g.stats.ms_walk = ms;
 }
 
-void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
-                             struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
-                             struct run_param* this_run) {
+static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
+                                         struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
+                                         struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
   if (!check_ready(cuda_mem, this_run)) return;
@@ -500,6 +502,20 @@ void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* 
   HIP_OK(hipStreamSynchronize(g.stream));
 }
 
+// The entry points whose host buffers grow with the caller's input report
+// a failed allocation as ARGOT_ERR_NOMEM: no exception crosses the C-ABI.
+void calc_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
+                             struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
+                             struct run_param* this_run) {
+  try {
+    calc_photoion_rate_cuda_body(cuda_mem, ray, ray_IO, this_cuda, this_run);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
+  }
+}
+
 void zero_out_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
                                  struct run_param* this_run) {
   (void)this_cuda;
@@ -509,8 +525,8 @@ void zero_out_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct cuda_pa
   HIP_OK(hipStreamSynchronize(g.stream));
 }
 
-void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
-                         struct cuda_param* this_cuda, struct run_param* this_run) {
+static void merge_photoion_data_body(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                                     struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
   if (!mesh || !check_ready(cuda_mem, this_run)) return;
@@ -529,6 +545,17 @@ void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_me
                         hipMemcpyHostToDevice, g.stream));
   HIP_OK(launch_add_rates(g.mesh, g.rate_tab.p, g.ncell, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                         struct cuda_param* this_cuda, struct run_param* this_run) {
+  try {
+    merge_photoion_data_body(mesh, cuda_mem, this_cuda, this_run);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
+  }
 }
 
 // One point-source pass of step_radiation_tree (calc_radiation_transfer
@@ -738,10 +765,10 @@ static void chem_step(const run_param* r, float dtime, float* diff_chem, float* 
   g.chem_stats.ms_kernel = ms;
 }
 
-void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
-                         struct run_param* this_run, struct mpi_param* this_mpi,
-                         struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
-                         float dtime) {
+static void step_radiation_tree_body(struct fluid_mesh* mesh, struct radiation_src* src,
+                                     struct run_param* this_run, struct mpi_param* this_mpi,
+                                     struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                                     float dtime) {
   (void)mesh;
   (void)this_mpi;
   (void)this_cuda;
@@ -791,6 +818,19 @@ void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
   g.stats.ms_kernel = ms_kernel;
   g.stats.ms_chem = ms_chem;
   g.stats.ms_total = now_ms() - t_start;
+}
+
+void step_radiation_tree(struct fluid_mesh* mesh, struct radiation_src* src,
+                         struct run_param* this_run, struct mpi_param* this_mpi,
+                         struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                         float dtime) {
+  try {
+    step_radiation_tree_body(mesh, src, this_run, this_mpi, cuda_mem, this_cuda, dtime);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
+  }
 }
 
 void step_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
@@ -857,9 +897,9 @@ void argot_hip_set_chem_queue(int n) { g_chem_queue = n > 0 ? std::min(n, 64) : 
 // src/smooth_photoion_rate.c:27-1588) on one rank: the rates in
 // mesh_dev[].prev_chem are smoothed on the device and, as the reference
 // does, also written into the host mesh's prev_chem (when mesh != NULL).
-void smooth_photoion_rate(struct fluid_mesh* mesh, struct run_param* this_run,
-                          struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
-                          struct mpi_param* this_mpi) {
+static void smooth_photoion_rate_body(struct fluid_mesh* mesh, struct run_param* this_run,
+                                      struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                                      struct mpi_param* this_mpi) {
   (void)this_cuda;
   (void)this_mpi;
   t_err = ARGOT_OK;
@@ -880,6 +920,18 @@ void smooth_photoion_rate(struct fluid_mesh* mesh, struct run_param* this_run,
     }
   } else {
     HIP_OK(hipStreamSynchronize(g.stream));
+  }
+}
+
+void smooth_photoion_rate(struct fluid_mesh* mesh, struct run_param* this_run,
+                          struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
+                          struct mpi_param* this_mpi) {
+  try {
+    smooth_photoion_rate_body(mesh, this_run, cuda_mem, this_cuda, this_mpi);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
   }
 }
 
@@ -1006,23 +1058,33 @@ int argot_hip_abi_layout(int64_t* out, int n) {
 
 // Product tree on the host: build from nsrc sources and list the rays of a
 // target point, for comparison with the CPU oracle.
+static void* argot_hip_tree_build_body(const float* pos, const double* spec, int nsrc, int ngrid,
+                                       const float* box6, float theta) {
+  std::unique_ptr<HostTree> t(new HostTree());
+  const TreeBox b{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]};
+  if (build_tree(pos, spec, (size_t)std::max(ngrid, 0), nsrc, ngrid, b, theta, t.get()) != 0)
+    return nullptr;
+  return t.release();
+}
+
 void* argot_hip_tree_build(const float* pos, const double* spec, int nsrc, int ngrid,
                            const float* box6, float theta) {
-  HostTree* t = new HostTree();
-  const TreeBox b{box6[0], box6[1], box6[2], box6[3], box6[4], box6[5]};
-  if (build_tree(pos, spec, (size_t)std::max(ngrid, 0), nsrc, ngrid, b, theta, t) != 0) {
-    delete t;
-    return nullptr;
+  try {
+    return argot_hip_tree_build_body(pos, spec, nsrc, ngrid, box6, theta);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
   }
-  return t;
+  return {};
 }
 
 void argot_hip_tree_free(void* h) { delete static_cast<HostTree*>(h); }
 
 int argot_hip_tree_nnode(void* h) { return static_cast<HostTree*>(h)->nnode(); }
 
-uint64_t argot_hip_tree_rays_to(void* h, const float* target, uint64_t cap, int* kind, int* id,
-                                float* xyz) {
+static uint64_t argot_hip_tree_rays_to_body(void* h, const float* target, uint64_t cap,
+                                            int* kind, int* id, float* xyz) {
   std::vector<HostRay> rays;
   const uint64_t n = tree_rays_to(*static_cast<HostTree*>(h), target, &rays);
   for (uint64_t i = 0; i < n && i < cap; i++) {
@@ -1033,6 +1095,18 @@ uint64_t argot_hip_tree_rays_to(void* h, const float* target, uint64_t cap, int*
     xyz[3 * i + 2] = rays[i].pos[2];
   }
   return n;
+}
+
+uint64_t argot_hip_tree_rays_to(void* h, const float* target, uint64_t cap, int* kind, int* id,
+                                float* xyz) {
+  try {
+    return argot_hip_tree_rays_to_body(h, target, cap, kind, id, xyz);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
+  }
+  return {};
 }
 
 const double* argot_hip_tree_spec(void* h, int node) {

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <new>
 #include <vector>
 
 #include "argot_hip.h"
@@ -633,8 +634,8 @@ using namespace argot;
 
 extern "C" {
 
-void argot_hip_surrogate_samples(const struct argot_surrogate_config* cfg, int64_t n,
-                                 float* log_tau, float* log_kernel) {
+static void argot_hip_surrogate_samples_body(const struct argot_surrogate_config* cfg, int64_t n,
+                                             float* log_tau, float* log_kernel) {
   argot_hip_internal_clear_error();
   S_ARG(cfg && n > 0 && log_tau && log_kernel && cfg->log_tau_lo < cfg->log_tau_hi);
   ThinHost th;
@@ -672,8 +673,19 @@ void argot_hip_surrogate_samples(const struct argot_surrogate_config* cfg, int64
   }
 }
 
-void argot_hip_surrogate_train(const struct argot_surrogate_config* cfg,
-                               struct argot_surrogate_report* rep) {
+void argot_hip_surrogate_samples(const struct argot_surrogate_config* cfg, int64_t n,
+                                 float* log_tau, float* log_kernel) {
+  try {
+    argot_hip_surrogate_samples_body(cfg, n, log_tau, log_kernel);
+  } catch (const std::bad_alloc&) {
+    argot_hip_internal_set_error(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    argot_hip_internal_set_error(ARGOT_ERR_ARG);
+  }
+}
+
+static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* cfg,
+                                           struct argot_surrogate_report* rep) {
   argot_hip_internal_clear_error();
   S_ARG(cfg && rep);
   S_ARG(cfg->batch >= 64 && cfg->batch % 64 == 0 && cfg->nsample >= cfg->batch &&
@@ -878,6 +890,17 @@ void argot_hip_surrogate_train(const struct argot_surrogate_config* cfg,
   s_.trained = true;
 }
 
+void argot_hip_surrogate_train(const struct argot_surrogate_config* cfg,
+                               struct argot_surrogate_report* rep) {
+  try {
+    argot_hip_surrogate_train_body(cfg, rep);
+  } catch (const std::bad_alloc&) {
+    argot_hip_internal_set_error(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    argot_hip_internal_set_error(ARGOT_ERR_ARG);
+  }
+}
+
 void argot_hip_surrogate_get_params(float* params, float* norm) {
   argot_hip_internal_clear_error();
   S_ARG(params && norm);
@@ -912,7 +935,7 @@ void argot_hip_surrogate_set_params(const float* params, const float* norm) {
   s_.trained = false;   // the trust grid belongs to the trained network
 }
 
-void argot_hip_surrogate_eval(const float* log_tau, float* log_kernel, int64_t n) {
+static void argot_hip_surrogate_eval_body(const float* log_tau, float* log_kernel, int64_t n) {
   argot_hip_internal_clear_error();
   S_ARG(log_tau && log_kernel && n > 0);
   if (!s_.have_net) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
@@ -933,8 +956,18 @@ void argot_hip_surrogate_eval(const float* log_tau, float* log_kernel, int64_t n
   S_OK(he);
 }
 
-void argot_hip_surrogate_gradient(const float* log_tau, const float* log_kernel, int64_t n,
-                                  float* grad, double* loss) {
+void argot_hip_surrogate_eval(const float* log_tau, float* log_kernel, int64_t n) {
+  try {
+    argot_hip_surrogate_eval_body(log_tau, log_kernel, n);
+  } catch (const std::bad_alloc&) {
+    argot_hip_internal_set_error(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    argot_hip_internal_set_error(ARGOT_ERR_ARG);
+  }
+}
+
+static void argot_hip_surrogate_gradient_body(const float* log_tau, const float* log_kernel,
+                                              int64_t n, float* grad, double* loss) {
   argot_hip_internal_clear_error();
   S_ARG(log_tau && log_kernel && grad && loss && n >= 64 && n % 64 == 0);
   if (!s_.have_net) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
@@ -984,6 +1017,17 @@ void argot_hip_surrogate_gradient(const float* log_tau, const float* log_kernel,
   for (void* p : {(void*)X, (void*)Y, (void*)partial, (void*)g, (void*)lp, (void*)lo})
     if (p) (void)hipFree(p);
   S_OK(he);
+}
+
+void argot_hip_surrogate_gradient(const float* log_tau, const float* log_kernel, int64_t n,
+                                  float* grad, double* loss) {
+  try {
+    argot_hip_surrogate_gradient_body(log_tau, log_kernel, n, grad, loss);
+  } catch (const std::bad_alloc&) {
+    argot_hip_internal_set_error(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    argot_hip_internal_set_error(ARGOT_ERR_ARG);
+  }
 }
 
 void argot_hip_surrogate_trust_error(float* err, int64_t n) {

@@ -454,7 +454,7 @@ enum argot_hip_status {
     ARGOT_ERR_ARG = 2,          /* bad argument or size */
     ARGOT_ERR_STATE = 3,        /* init_gpu not called / mesh not sent /
                                    rate mode 1 without a trained surrogate */
-    ARGOT_ERR_NOMEM = 4,        /* device allocation failed */
+    ARGOT_ERR_NOMEM = 4,        /* device or host allocation failed */
     ARGOT_ERR_TOO_MANY_SEGMENTS = 5, /* ray crosses too many domains */
     ARGOT_ERR_IO = 6            /* a file could not be opened, or is short */
 };
