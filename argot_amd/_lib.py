@@ -20,7 +20,7 @@ EXPORTS = [
     "allocate_pinned_segment", "deallocate_pinned_segment",
     "allocate_pinned_light_ray_IO", "deallocate_pinned_light_ray_IO",
     "calc_optical_depth_cuda", "calc_photoion_rate_cuda", "zero_out_photoion_rate_cuda",
-    "merge_photoion_data", "step_radiation_tree",
+    "merge_photoion_data", "recv_photoion_data", "send_photoion_data", "step_radiation_tree",
     "argot_hip_last_error", "argot_hip_error_string", "argot_hip_set_helium_fact",
     "argot_hip_set_tau_mode", "argot_hip_set_rate_mode", "argot_hip_step_stats",
     "argot_hip_set_theta", "argot_hip_set_blocks_per_cu", "argot_hip_copy_probe",
@@ -72,6 +72,8 @@ def lib():
     L.calc_photoion_rate_cuda.argtypes = [cm, C.c_void_p, C.c_void_p, cp, rp]
     L.zero_out_photoion_rate_cuda.argtypes = [cm, cp, rp]
     L.merge_photoion_data.argtypes = [C.c_void_p, cm, cp, rp]
+    L.recv_photoion_data.argtypes = [C.c_void_p, cm, cp, rp]
+    L.send_photoion_data.argtypes = [C.c_void_p, cm, cp, rp]
     L.step_radiation_tree.argtypes = [C.c_void_p, C.c_void_p, rp, C.c_void_p, cm, cp, C.c_float]
     L.argot_hip_last_error.restype = C.c_int
     L.argot_hip_error_string.argtypes = [C.c_int]

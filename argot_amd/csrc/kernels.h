@@ -24,6 +24,9 @@ hipError_t launch_pack_fields(const void* mesh, int64_t ncell, double helium_fac
                               hipStream_t s);
 hipError_t launch_zero_rates(void* mesh, int64_t ncell, hipStream_t s);
 hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s);
+// rate[6c + k] := prev_chem rate k of cell c, and back
+hipError_t launch_get_rates(const void* mesh, float* rate, int64_t ncell, hipStream_t s);
+hipError_t launch_set_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s);
 // wide: address the cell fields in 64 bits even when the mesh has fewer
 // than 2^28 cells (meshes with more always are).
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,

@@ -62,6 +62,28 @@ __global__ void add_rates_kernel(unsigned char* __restrict__ mesh, const float* 
   for (int k = 0; k < 6; k++) r[k] = r[k] + rate[6 * c + k];
 }
 
+// The six rates of every cell between the mesh and a photoion_rate table
+// (24 B per cell, the prev_chem rates in the same order):
+// recv_photoion_data_kernel / send_photoion_data_kernel of
+// src/recv_photoion_data.cu and src/send_photoion_data.cu.
+__global__ void get_rates_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
+                                 float* __restrict__ rate) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const float* r = mesh_f(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) rate[6 * c + k] = r[k];
+}
+
+__global__ void set_rates_kernel(unsigned char* __restrict__ mesh, const float* __restrict__ rate,
+                                 int64_t ncell) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  float* r = mesh_fw(mesh, c, kOffRates);
+#pragma unroll
+  for (int k = 0; k < 6; k++) r[k] = rate[6 * c + k];
+}
+
 // ray_segment records (56 B): start @16, end @28, tau @40.  A block takes
 // 256 consecutive segments and hands them to its lanes in order of their
 // estimated crossing count (counting sort in LDS), so that the lanes of a
@@ -729,6 +751,20 @@ hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, 
   if (nray <= 0) return hipSuccess;
   merge_ray_rates_kernel<<<nblocks(nray, 256), 256, 0, s>>>(per_ray, target, nray,
                                                              static_cast<unsigned char*>(mesh));
+  return hipGetLastError();
+}
+
+hipError_t launch_get_rates(const void* mesh, float* rate, int64_t ncell, hipStream_t s) {
+  if (ncell <= 0) return hipSuccess;
+  get_rates_kernel<<<nblocks(ncell, 256), 256, 0, s>>>(static_cast<const unsigned char*>(mesh),
+                                                      ncell, rate);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_rates(void* mesh, const float* rate, int64_t ncell, hipStream_t s) {
+  if (ncell <= 0) return hipSuccess;
+  set_rates_kernel<<<nblocks(ncell, 256), 256, 0, s>>>(static_cast<unsigned char*>(mesh), rate,
+                                                      ncell);
   return hipGetLastError();
 }
 

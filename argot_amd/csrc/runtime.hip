@@ -547,6 +547,34 @@ static void merge_photoion_data_body(struct fluid_mesh* mesh, struct cuda_mem_sp
   HIP_OK(hipStreamSynchronize(g.stream));
 }
 
+void recv_photoion_data(struct photoion_rate* gamma, struct cuda_mem_space* cuda_mem,
+                        struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  if (!gamma) return set_err(ARGOT_ERR_ARG);
+  if (!cuda_mem->rate_dev) return set_err(ARGOT_ERR_STATE);
+  float* rate = reinterpret_cast<float*>(cuda_mem->rate_dev);
+  HIP_OK(launch_get_rates(g.mesh, rate, g.ncell, g.stream));
+  HIP_OK(hipMemcpyAsync(gamma, rate, sizeof(struct photoion_rate) * g.ncell,
+                        hipMemcpyDeviceToHost, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
+void send_photoion_data(struct photoion_rate* gamma, struct cuda_mem_space* cuda_mem,
+                        struct cuda_param* this_cuda, struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  if (!check_ready(cuda_mem, this_run)) return;
+  if (!gamma) return set_err(ARGOT_ERR_ARG);
+  if (!cuda_mem->rate_dev) return set_err(ARGOT_ERR_STATE);
+  float* rate = reinterpret_cast<float*>(cuda_mem->rate_dev);
+  HIP_OK(hipMemcpyAsync(rate, gamma, sizeof(struct photoion_rate) * g.ncell,
+                        hipMemcpyHostToDevice, g.stream));
+  HIP_OK(launch_set_rates(g.mesh, rate, g.ncell, g.stream));
+  HIP_OK(hipStreamSynchronize(g.stream));
+}
+
 void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
                          struct cuda_param* this_cuda, struct run_param* this_run) {
   try {

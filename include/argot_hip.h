@@ -345,6 +345,14 @@ void argot_hip_set_chem_queue(int n);
  * cells, 64-bit ones otherwise.  Results are identical (a test control for
  * the large-mesh path). */
 void argot_hip_set_wide_index(int on);
+/* replaces recv_photoion_data, src/recv_photoion_data.cu:14-43:
+ * gamma[c] := mesh_dev[c].prev_chem rates (through cuda_mem->rate_dev). */
+void recv_photoion_data(struct photoion_rate *gamma, struct cuda_mem_space *cuda_mem,
+                        struct cuda_param *this_cuda, struct run_param *this_run);
+/* replaces send_photoion_data, src/send_photoion_data.cu:12-41:
+ * mesh_dev[c].prev_chem rates := gamma[c]. */
+void send_photoion_data(struct photoion_rate *gamma, struct cuda_mem_space *cuda_mem,
+                        struct cuda_param *this_cuda, struct run_param *this_run);
 /* replaces merge_photoion_data, src/merge_photoion_data.cu:16-83:
  * mesh_dev[].prev_chem += host mesh[].prev_chem rates. */
 void merge_photoion_data(struct fluid_mesh *mesh,

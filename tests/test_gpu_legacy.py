@@ -158,3 +158,32 @@ def test_walk_entry_point_equals_fused_walk(require_gpu):
         L.deallocate_pinned_segment(segp)
     assert np.array_equal(tau.view(np.uint32), rec["tau"].view(np.uint32))
     assert step.stats()["ms_walk"] > 0.0
+
+
+def test_photoion_data_round_trip(require_gpu):
+    """recv_photoion_data gives the mesh's six rates per cell; send_photoion_data
+    puts a table back into the device mesh and touches nothing else."""
+    w = small_workload(n=12, nsrc=4, seed=41)
+    step = R.PointSourceStep(w)
+    step.step()
+    want = step.rates()
+    L = lib()
+    n = w.n ** 3
+    gamma = (abi.PhotoionRate * n)()
+    L.recv_photoion_data(gamma, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+    check("recv_photoion_data")
+    got = np.ctypeslib.as_array(gamma).view(np.float32).reshape(n, 6)
+    np.testing.assert_array_equal(got, want)
+    before = abi.mesh_view(step.mesh).copy()
+    new = (np.arange(6 * n, dtype=np.float32).reshape(n, 6) + 0.5)
+    np.ctypeslib.as_array(gamma).view(np.float32).reshape(n, 6)[:] = new
+    L.send_photoion_data(gamma, C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+    check("send_photoion_data")
+    np.testing.assert_array_equal(step.rates(), new)
+    after = abi.mesh_view(step.mesh)
+    for f in after.dtype.names:
+        if f != "prev_chem":
+            assert after[f].tobytes() == before[f].tobytes(), f
+    for f in after["prev_chem"].dtype.names:
+        if f not in R.RATE_FIELDS:
+            assert np.array_equal(after["prev_chem"][f], before["prev_chem"][f]), f
