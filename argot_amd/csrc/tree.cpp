@@ -239,8 +239,18 @@ int build_tree(const float* pos, const double* spec, size_t spec_stride, int nsr
   t->spec_scale.assign(nn, 0.0);
   for (int u = 0; u < nn; u++) {
     const double* row = &t->spec64[(size_t)u * ngrid];
-    double mx = 0.0;
-    for (int k = 0; k < ngrid; k++) mx = std::max(mx, row[k]);
+    // four independent running maxima: max is exact, so splitting the
+    // comparisons leaves the result unchanged and breaks the serial chain
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    int k = 0;
+    for (; k + 4 <= ngrid; k += 4) {
+      m0 = std::max(m0, row[k]);
+      m1 = std::max(m1, row[k + 1]);
+      m2 = std::max(m2, row[k + 2]);
+      m3 = std::max(m3, row[k + 3]);
+    }
+    for (; k < ngrid; k++) m0 = std::max(m0, row[k]);
+    const double mx = std::max(std::max(m0, m1), std::max(m2, m3));
     t->spec_scale[u] = mx;
     const double inv = mx > 0.0 ? 1.0 / mx : 0.0;
     for (int k = 0; k < ngrid; k++) t->spec[(size_t)u * ngrid + k] = (float)(row[k] * inv);
