@@ -30,6 +30,7 @@ struct Build {
   std::vector<uint64_t> key;
   std::vector<int> order;   // sorted source order
   std::vector<double> scratch;  // [ngrid]: the root's children sum (unused)
+  std::vector<double> lum;      // [nsrc]: each source's sum over the bins
   float base_length;
   HostTree* out;
 };
@@ -148,8 +149,7 @@ Sums plant(Build& b, int self, int first, int n, int level, int out) {
       for (int j = beg[c]; j < beg[c] + cnt[c]; j++) {
         const int s = b.order[j];
         const double* ps = b.spec + (size_t)s * b.stride;
-        double m = 0.0;
-        for (int k = 0; k < ng; k++) m += ps[k];
+        const double m = b.lum[s];
         for (int k = 0; k < ng; k++) sub[k] += ps[k];
         mt += m;
         for (int d = 0; d < 3; d++) mp[d] += m * b.pos[3 * s + d];
@@ -204,6 +204,35 @@ int build_tree(const float* pos, const double* spec, size_t spec_stride, int nsr
     b.key[i] = interleave(ix, iy, iz);
   }
   if (nsrc > 0) sort_keys(b.order.data(), b.key.data(), 0, nsrc - 1);
+  // Each source's luminosity, its bins summed in order; four sources at a
+  // time, so that four independent chains of additions overlap.
+  b.lum.assign(std::max(nsrc, 1), 0.0);
+  {
+    int s = 0;
+    for (; s + 4 <= nsrc; s += 4) {
+      const double* p0 = spec + (size_t)s * spec_stride;
+      const double* p1 = p0 + spec_stride;
+      const double* p2 = p1 + spec_stride;
+      const double* p3 = p2 + spec_stride;
+      double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+      for (int k = 0; k < ngrid; k++) {
+        l0 += p0[k];
+        l1 += p1[k];
+        l2 += p2[k];
+        l3 += p3[k];
+      }
+      b.lum[s] = l0;
+      b.lum[s + 1] = l1;
+      b.lum[s + 2] = l2;
+      b.lum[s + 3] = l3;
+    }
+    for (; s < nsrc; s++) {
+      const double* ps = spec + (size_t)s * spec_stride;
+      double l = 0.0;
+      for (int k = 0; k < ngrid; k++) l += ps[k];
+      b.lum[s] = l;
+    }
+  }
   const size_t nnode = 1 + (nsrc > 0 ? (size_t)count_nodes(b.key.data(), 0, nsrc, 63) : 0);
   t->link.reserve(nnode);
   t->cm.reserve(4 * nnode);
