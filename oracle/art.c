@@ -234,6 +234,18 @@ static double k06_B(double t)
     return 1.3765e-14 * pow(lambda, 1.5) / pow(1.0 + pow(lambda / 2.740, 0.407), 2.242);
 }
 
+/* The six recombination fits at temperature t (tests): k02_A, k02_B,
+ * k04_A, k04_B, k06_A, k06_B. */
+void orc_art_recomb(double t, double *out)
+{
+    out[0] = k02_A(t);
+    out[1] = k02_B(t);
+    out[2] = k04_A(t);
+    out[3] = k04_B(t);
+    out[4] = k06_A(t);
+    out[5] = k06_B(t);
+}
+
 void orc_art_rmesh(long ncell, const float *dens, const float *chem, double denstonh,
                    double uenetok, double helium_fact, float *kap, float *src)
 {

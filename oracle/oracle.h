@@ -85,6 +85,8 @@ int orc_tree_nnode(const orc_tree *t);
 /* Node record access (for tests): cm, length, n, spectrum row. */
 void orc_tree_node(const orc_tree *t, int node, float *cm, float *length,
                    int *n, int *nchild, const double **spec);
+/* Sources below node k (original indices, sorted-key order); returns n. */
+int orc_tree_node_members(const orc_tree *t, int node, int *out);
 
 /* count_ray_to, src/setup_light_ray_tree.c:587-605 + count_src :241-283 */
 uint64_t orc_count_ray_to(const orc_tree *t, const orc_grid *g,
@@ -203,6 +205,9 @@ void orc_art_step_factors(int heatcool, float f[6]);
 
 /* calc_rmesh_data: chem[7*c] = felec, fHI, fHII, fHeI, fHeII, fHeIII,
  * prev_uene; kap/src [3*c] for the three bands. */
+/* The recombination fits of src/diffuse_photon/diffuse_chemistry.c:13-97
+ * at temperature t: k02_A, k02_B, k04_A, k04_B, k06_A, k06_B. */
+void orc_art_recomb(double t, double *out);
 void orc_art_rmesh(long ncell, const float *dens, const float *chem, double denstonh,
                    double uenetok, double helium_fact, float *kap, float *src);
 

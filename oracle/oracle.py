@@ -11,9 +11,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# The same restatement built with the reference's own Makefile flags
+# (-O3 -ffast-math -funroll-loops, src/Makefile:69): timed beside the IEEE
+# build as the CPU baseline; never the checker.
+LIB_FAST_PATH = os.path.join(HERE, "liboracle_fast.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
-_lib = None
+_libs = {}
 
 f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -46,14 +50,15 @@ class Stats(C.Structure):
     _fields_ = [("nray", C.c_uint64), ("ncross", C.c_uint64), ("nseg", C.c_uint64)]
 
 
-def lib():
-    """Load liboracle.so (built by `make -C oracle`)."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError("oracle/liboracle.so is not built: run `make -C oracle`")
-    L = C.CDLL(LIB_PATH)
+def lib(fast=False):
+    """Load liboracle.so (built by `make -C oracle`), or with fast=True the
+    build at the reference's compiler flags (liboracle_fast.so)."""
+    if fast in _libs:
+        return _libs[fast]
+    path = LIB_FAST_PATH if fast else LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `make -C oracle`")
+    L = C.CDLL(path)
     L.orc_freq_grid.argtypes = [C.c_int, C.c_double, C.c_double, f64p, C.POINTER(C.c_double)]
     L.orc_photon_rate_bb.argtypes = [C.c_int, C.c_double, C.c_double, C.c_float, C.c_double, f64p]
     for n in ("orc_csect_HI", "orc_csect_HeI", "orc_csect_HeII"):
@@ -64,6 +69,8 @@ def lib():
     L.orc_tree_build.restype = C.c_void_p
     L.orc_tree_free.argtypes = [C.c_void_p]
     L.orc_tree_nnode.argtypes = [C.c_void_p]
+    L.orc_tree_node_members.argtypes = [C.c_void_p, C.c_int, i32p]
+    L.orc_tree_node_members.restype = C.c_int
     L.orc_tree_node.argtypes = [C.c_void_p, C.c_int, f32p, C.POINTER(C.c_float),
                                 C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.POINTER(C.c_double))]
@@ -95,6 +102,7 @@ def lib():
     L.orc_pix2ang_ring.argtypes = [C.c_long, C.c_long, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]
     L.orc_art_angles.argtypes = [C.c_int, C.c_int, f32p, i16p, i16p, i32p]
+    L.orc_art_recomb.argtypes = [C.c_double, f64p]
     L.orc_art_step_factors.argtypes = [C.c_int, f32p]
     L.orc_art_rmesh.argtypes = [C.c_long, f32p, f32p, C.c_double, C.c_double, C.c_double,
                                 f32p, f32p]
@@ -103,7 +111,7 @@ def lib():
     L.orc_art_gh_tot.argtypes = [C.c_long, f32p, f32p, f32p, f32p, f32p, f32p]
     L.orc_art_run.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, f32p, f32p, C.c_double, f32p]
     L.orc_art_run.restype = C.c_long
-    _lib = L
+    _libs[fast] = L
     return L
 
 
@@ -131,8 +139,8 @@ class Problem:
     """A mesh + sources in the oracle's terms (box [0,1)^3 by default)."""
 
     def __init__(self, dens, fHI, fHeI, fHeII, src_pos, src_spec, *, lunit, denstonh,
-                 helium_fact, theta=0.7, nu=None, cs=None, box=(0.0, 1.0)):
-        L = lib()
+                 helium_fact, theta=0.7, nu=None, cs=None, box=(0.0, 1.0), fast=False):
+        L = self._L = lib(fast)
         self.shape = dens.shape
         nx, ny, nz = dens.shape
         self.dens = np.ascontiguousarray(dens, dtype=np.float32)
@@ -172,7 +180,7 @@ class Problem:
     def __del__(self):
         try:
             if self.tree:
-                lib().orc_tree_free(self.tree)
+                self._L.orc_tree_free(self.tree)
         except Exception:
             pass
 
@@ -181,10 +189,10 @@ class Problem:
         return int(np.prod(self.shape))
 
     def count_ray_to(self, ix, iy, iz):
-        return int(lib().orc_count_ray_to(self.tree, C.byref(self.grid), ix, iy, iz))
+        return int(self._L.orc_count_ray_to(self.tree, C.byref(self.grid), ix, iy, iz))
 
     def rays_to(self, ix, iy, iz):
-        L = lib()
+        L = self._L
         n = self.count_ray_to(ix, iy, iz)
         kind = np.zeros(max(n, 1), np.int32)
         ids = np.zeros(max(n, 1), np.int32)
@@ -193,25 +201,31 @@ class Problem:
         return kind[:n], ids[:n], pos[:3 * n].reshape(-1, 3)
 
     def ray_spec(self, kind, ident):
-        p = lib().orc_ray_spec(self.tree, int(kind), int(ident))
+        p = self._L.orc_ray_spec(self.tree, int(kind), int(ident))
         return np.ctypeslib.as_array(p, shape=(self.grid.ngrid,)).copy()
 
     def node(self, k):
         cm = np.zeros(3, np.float32)
         length, n, nch = C.c_float(), C.c_int(), C.c_int()
         sp = C.POINTER(C.c_double)()
-        lib().orc_tree_node(self.tree, k, cm, C.byref(length), C.byref(n), C.byref(nch),
+        self._L.orc_tree_node(self.tree, k, cm, C.byref(length), C.byref(n), C.byref(nch),
                             C.byref(sp))
         spec = np.ctypeslib.as_array(sp, shape=(self.grid.ngrid,)).copy()
         return cm, length.value, n.value, nch.value, spec
 
+    def node_members(self, k):
+        """Original indices of the sources below node k (sorted-key order)."""
+        out = np.zeros(max(self.nsrc, 1), np.int32)
+        n = self._L.orc_tree_node_members(self.tree, k, out)
+        return out[:n].copy()
+
     def nnode(self):
-        return int(lib().orc_tree_nnode(self.tree))
+        return int(self._L.orc_tree_nnode(self.tree))
 
     def ray_segment(self, src, ix, iy, iz, maxseg=8):
         s = np.zeros(3 * maxseg, np.float32)
         e = np.zeros(3 * maxseg, np.float32)
-        n = lib().orc_ray_segment(C.byref(self.grid), np.ascontiguousarray(src, np.float32),
+        n = self._L.orc_ray_segment(C.byref(self.grid), np.ascontiguousarray(src, np.float32),
                                   ix, iy, iz, maxseg, s, e)
         return s[:3 * n].reshape(-1, 3), e[:3 * n].reshape(-1, 3)
 
@@ -219,7 +233,7 @@ class Problem:
         tau = np.zeros(3, np.float32)
         cap = sum(self.shape) + 8
         buf = np.zeros(cap, np.int32)
-        n = lib().orc_walk_fp32(C.byref(self.grid), C.byref(self.fields),
+        n = self._L.orc_walk_fp32(C.byref(self.grid), C.byref(self.fields),
                                 np.ascontiguousarray(start, np.float32),
                                 np.ascontiguousarray(end, np.float32), tau, cap,
                                 buf.ctypes.data_as(C.c_void_p))
@@ -229,24 +243,24 @@ class Problem:
 
     def walk_kahan(self, start, end):
         tau = np.zeros(3, np.float32)
-        n = lib().orc_walk_fp32_kahan(C.byref(self.grid), C.byref(self.fields),
+        n = self._L.orc_walk_fp32_kahan(C.byref(self.grid), C.byref(self.fields),
                                       np.ascontiguousarray(start, np.float32),
                                       np.ascontiguousarray(end, np.float32), tau)
         return tau, n
 
     def walk_exact(self, start, end):
         tau = np.zeros(3, np.float64)
-        n = lib().orc_walk_exact(C.byref(self.grid), C.byref(self.fields),
+        n = self._L.orc_walk_exact(C.byref(self.grid), C.byref(self.fields),
                                  np.ascontiguousarray(start, np.float32),
                                  np.ascontiguousarray(end, np.float32), tau)
         return tau, n
 
     def delta_length(self, dx, dy, dz):
-        return lib().orc_delta_length(C.byref(self.grid), dx, dy, dz)
+        return self._L.orc_delta_length(C.byref(self.grid), dx, dy, dz)
 
     def ray_rate(self, src, spec, ix, iy, iz, tau, mode=0):
         out = np.zeros(6)
-        lib().orc_ray_rate(C.byref(self.grid), C.byref(self.fields),
+        self._L.orc_ray_rate(C.byref(self.grid), C.byref(self.fields),
                            np.ascontiguousarray(src, np.float32),
                            np.ascontiguousarray(spec, np.float64), ix, iy, iz,
                            np.ascontiguousarray(tau, np.float64), mode, out)
@@ -259,7 +273,7 @@ class Problem:
         r32 = np.zeros(6 * self.ncell, np.float32)
         r64 = np.zeros(6 * self.ncell, np.float64) if f64 else None
         st = Stats()
-        lib().orc_step(C.byref(self.grid), C.byref(self.fields), self.tree, t0, t1, r32,
+        self._L.orc_step(C.byref(self.grid), C.byref(self.fields), self.tree, t0, t1, r32,
                        r64.ctypes.data_as(C.c_void_p) if f64 else None, int(kahan),
                        int(nthreads), C.byref(st))
         return (r32.reshape(-1, 6), None if r64 is None else r64.reshape(-1, 6),
@@ -267,7 +281,7 @@ class Problem:
 
     def ray_records(self, t0, t1):
         """Per-ray (target, kind, id, tau[3], ncross, cell-sequence hash)."""
-        L = lib()
+        L = self._L
         n = sum(self.count_ray_to(*cell_index(im, self.shape)) for im in range(t0, t1))
         m = max(n, 1)
         tgt = np.zeros(m, np.int64)
@@ -480,6 +494,32 @@ def chem_dtime_mesh(mesh_bytes, p):
 def chem_update_mesh(mesh_bytes):
     m = mesh_bytes.reshape(-1)
     _chem_lib().orc_chem_update_mesh(m, m.size // 156)
+
+
+REF_ART = os.path.join(REF_DIR, "libref_art.so")
+_ref_art = None
+
+
+def ref_art():
+    """The reference's own diffuse-photon recombination fits
+    (oracle/_ref/libref_art.so), or None when it is not built."""
+    global _ref_art
+    if _ref_art is not None:
+        return _ref_art
+    if not os.path.exists(REF_ART):
+        return None
+    L = C.CDLL(REF_ART)
+    L.ref_art_recomb.argtypes = [C.c_double, f64p]
+    _ref_art = L
+    return L
+
+
+def art_recomb(t):
+    """The oracle's recombination fits at t [K]: k02_A, k02_B, k04_A,
+    k04_B, k06_A, k06_B."""
+    out = np.zeros(6)
+    lib().orc_art_recomb(float(t), out)
+    return out
 
 
 REF_CHEM = os.path.join(REF_DIR, "libref_chem.so")

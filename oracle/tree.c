@@ -245,6 +245,16 @@ void orc_tree_free(orc_tree *t)
 
 int orc_tree_nnode(const orc_tree *t) { return t->nnode; }
 
+/* The sources below a node (original indices, in sorted-key order):
+ * index[ifirst .. ifirst + n) of the sorted source list, the range the
+ * reference's clist record names (clist[].ifirst, clist[].n). */
+int orc_tree_node_members(const orc_tree *t, int node, int *out)
+{
+    const onode *p = &t->node[node];
+    for (int j = 0; j < p->n; j++) out[j] = t->index[p->ifirst + j];
+    return p->n;
+}
+
 void orc_tree_node(const orc_tree *t, int node, float *cm, float *length,
                    int *n, int *nchild, const double **spec)
 {

@@ -104,6 +104,25 @@ def test_recombination_fits_match_textbook_coefficients():
     assert abs(kap[0] / (6.3e-18 * ch[1] * nH) - 1) < 1e-5
 
 
+def test_recombination_fits_equal_the_reference_build():
+    """The six fits of src/diffuse_photon/diffuse_chemistry.c:13-97 (k02,
+    k04, k06, case A and B) bit for bit against the reference's own file
+    compiled unmodified (oracle/_ref/libref_art.so), over 10 K to 1e9 K
+    including both sides of k04_A's 0.8 eV and k04_B's 500.3 K branches."""
+    ref = O.ref_art()
+    if ref is None:
+        pytest.skip("oracle/_ref/libref_art.so not built")
+    ts = np.concatenate([np.logspace(1.0, 9.0, 4001),
+                         [500.3, np.nextafter(500.3, 0), np.nextafter(500.3, 1e9),
+                          0.8 / 8.61623e-5, np.nextafter(0.8 / 8.61623e-5, 0),
+                          np.nextafter(0.8 / 8.61623e-5, 1e9), 1.0e4, 157807.0]])
+    out = np.zeros(6)
+    for t in ts:
+        ref.ref_art_recomb(float(t), out)
+        mine = O.art_recomb(t)
+        assert np.array_equal(mine.view(np.uint64), out.view(np.uint64)), (t, mine, out)
+
+
 @pytest.mark.parametrize("heatcool", [False, True])
 def test_step_factors_against_quadrature(heatcool):
     f = O.art_step_factors(heatcool)

@@ -22,8 +22,14 @@ import pytest
 import oracle as O
 from argot_amd import ic
 
-REF = O.ref_chem()
-needs_ref = pytest.mark.skipif(REF is None, reason="oracle/_ref/libref_chem.so not built")
+@pytest.fixture(scope="module")
+def REF():
+    """The reference's CPU chemistry build, loaded when a test asks for it
+    (not at collection), skipped where it is not built."""
+    L = O.ref_chem()
+    if L is None:
+        pytest.skip("oracle/_ref/libref_chem.so not built")
+    return L
 
 KTOEV = 8.61623e-5
 
@@ -58,25 +64,22 @@ def temps():
     return t
 
 
-@needs_ref
-def test_rate_fits_match_the_reference_build():
+def test_rate_fits_match_the_reference_build(REF):
     for t in temps():
         k, r = O.chem_k(t), np.zeros(6)
         REF.ref_chem_k(t, r)
         assert np.array_equal(k.view(np.uint64), r.view(np.uint64)), t
 
 
-@needs_ref
 @pytest.mark.parametrize("zred", [0.0, 3.5])
-def test_cooling_functions_match_the_reference_build(zred):
+def test_cooling_functions_match_the_reference_build(REF, zred):
     for t in temps():
         o, r = O.chem_cooling(t, zred), np.zeros(12)
         REF.ref_chem_cooling(t, zred, r)
         assert np.array_equal(o.view(np.uint64), r.view(np.uint64)), (t, O.COOLING_NAMES)
 
 
-@needs_ref
-def test_heat_minus_cool_matches_the_reference_build():
+def test_heat_minus_cool_matches_the_reference_build(REF):
     rng = np.random.default_rng(7)
     hf = REF.ref_chem_helium_fact()
     for _ in range(400):
@@ -88,8 +91,7 @@ def test_heat_minus_cool_matches_the_reference_build():
         assert a == b or (np.isnan(a) and np.isnan(b)), (a, b)
 
 
-@needs_ref
-def test_energy_update_matches_the_reference_build():
+def test_energy_update_matches_the_reference_build(REF):
     rng = np.random.default_rng(8)
     hf = REF.ref_chem_helium_fact()
     p, uv = units(hf)
@@ -105,8 +107,7 @@ def test_energy_update_matches_the_reference_build():
         assert u1 == np.float32(ur.value), (i, u1, ur.value)
 
 
-@needs_ref
-def test_network_step_tracks_the_reference_cpu_path():
+def test_network_step_tracks_the_reference_cpu_path(REF):
     """GPU arithmetic (float densities) against the reference's CPU code
     (double densities) on the same cells: the sub-step counts, fractions and
     energies agree to a few 1e-4; a missing source or sink term would move
@@ -132,8 +133,7 @@ def test_network_step_tracks_the_reference_cpu_path():
     assert worst[6] < 1e-3, worst
 
 
-@needs_ref
-def test_safe_step_tracks_the_reference_cpu_path():
+def test_safe_step_tracks_the_reference_cpu_path(REF):
     rng = np.random.default_rng(10)
     hf = REF.ref_chem_helium_fact()
     p, uv = units(hf)

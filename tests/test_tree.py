@@ -34,10 +34,113 @@ def centres(ix, iy, iz, n):
     return np.array([cell_centre(ix, n), cell_centre(iy, n), cell_centre(iz, n)], np.float32)
 
 
-@pytest.mark.parametrize("name", ["small", "clumped"])
+def mixed_workload(n=16, nsrc=120, seed=41, lo=0.0, hi=1.0):
+    """Sources of two blackbody temperatures (3e4 K and 1e5 K), so that a
+    centre weighted by each source's summed spectrum (spec_to_L) differs
+    from one weighted by its first bin (the root's weight) or unweighted."""
+    w = small_workload(n=n, nsrc=nsrc, seed=seed, inside_cells=False)
+    rng = np.random.default_rng(seed + 1)
+    w.src_pos = rng.uniform(lo, hi, (nsrc, 3)).astype(np.float32)
+    w.src_temp = rng.choice(np.array([3.0e4, 1.0e5], np.float32), nsrc)
+    return w
+
+
+def morton_keys(pos, lo=0.0, base=1.0):
+    """63-bit Morton keys of construct_tree (setup_light_ray_tree.c:488-515):
+    (int)(float((x - xmin) / base_length) * 2^21) per axis, bit b of x, y, z
+    at bits 3b+2, 3b+1, 3b, with bit 63 set."""
+    q = []
+    for d in range(3):
+        v = (np.float32(pos[:, d]) - np.float32(lo)) / np.float32(base)
+        q.append(np.trunc(v.astype(np.float32).astype(np.float64) * float(1 << 21)).astype(np.int64))
+    keys = []
+    for i in range(len(pos)):
+        k = 1 << 63
+        for b in range(21):
+            k |= ((((int(q[0][i]) >> b) & 1) << 2) | (((int(q[1][i]) >> b) & 1) << 1) |
+                  ((int(q[2][i]) >> b) & 1)) << (3 * b)
+        keys.append(k)
+    return keys
+
+
+def ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+
+
+@pytest.mark.parametrize("lo,hi", [(0.0, 1.0), (-0.6, 1.7)])
+def test_node_centres_brute_force(lo, hi):
+    """Every node of the oracle's tree against numpy brute force: its member
+    set is exactly the sources whose Morton keys share its prefix, its
+    spectrum is their summed spectra, and its centre is their mean weighted
+    by spec_to_L (each source's spectrum summed over the bins,
+    setup_light_ray_tree.c:207-224); the root's centre is weighted by
+    photon_rate[0] instead (:529-545).  With two temperatures the weights
+    differ, so a wrong weight moves centres by many ulps."""
+    w = mixed_workload(lo=lo, hi=hi)
+    P = oracle_problem(w)
+    pos = P.src_pos.reshape(-1, 3).astype(np.float64)
+    spec = P.src_spec.reshape(P.nsrc, -1)
+    lum = spec.sum(1)
+    keys = morton_keys(P.src_pos.reshape(-1, 3))
+    nn = P.nnode()
+    temp = np.asarray(w.src_temp)
+    wrong_weight_ulps = []
+    for k in range(nn):
+        cm, length, n, nch, sp = P.node(k)
+        mem = P.node_members(k)
+        assert len(mem) == n
+        depth = int(round(-np.log2(length)))
+        assert length == 2.0 ** -depth
+        shift = 63 - 3 * depth
+        pref = {keys[i] >> shift for i in mem}
+        assert len(pref) == 1
+        p = pref.pop()
+        assert sorted(mem.tolist()) == [i for i in range(P.nsrc) if keys[i] >> shift == p]
+        np.testing.assert_allclose(sp, spec[mem].sum(0), rtol=1e-12)
+        wgt = spec[mem, 0] if k == 0 else lum[mem]
+        ref = (wgt[:, None] * pos[mem]).sum(0) / wgt.sum()
+        assert ulps(cm, ref).max() <= 1, (k, cm, ref)
+        other = lum[mem] if k == 0 else spec[mem, 0]
+        alt = (other[:, None] * pos[mem]).sum(0) / other.sum()
+        if len(set(temp[mem].tolist())) > 1:
+            wrong_weight_ulps.append(ulps(cm, alt).max())
+    # the test discriminates: on nodes holding both temperatures the other
+    # weighting is far off
+    assert len(wrong_weight_ulps) > 20
+    assert np.mean(np.array(wrong_weight_ulps) > 100) > 0.9
+
+
+def test_ray_order_is_morton_order():
+    """theta -> 0 opens every node, so a target's rays are the sources in
+    tree pre-order: ascending Morton key (distinct keys here), in the oracle
+    and in the library's tree alike."""
+    w = mixed_workload(nsrc=80, seed=8, lo=-0.3, hi=1.3)
+    P = oracle_problem(w, theta=1e-6)
+    keys = morton_keys(P.src_pos.reshape(-1, 3))
+    assert len(set(keys)) == len(keys)
+    order = sorted(range(P.nsrc), key=lambda i: keys[i])
+    h = product_tree(P, theta=1e-6)
+    try:
+        for ix, iy, iz in [(0, 0, 0), (5, 9, 2), (15, 15, 15), (7, 0, 12)]:
+            kind, ids, pos = P.rays_to(ix, iy, iz)
+            assert np.all(kind == 1)
+            assert ids.tolist() == order
+            kp, ip, pp = product_rays(h, centres(ix, iy, iz, w.n))
+            assert np.all(kp == 1) and ip.tolist() == order
+    finally:
+        lib().argot_hip_tree_free(h)
+
+
+@pytest.mark.parametrize("name", ["small", "clumped", "mixed", "outside"])
 def test_product_tree_matches_oracle(name):
     if name == "small":
         w = small_workload(n=16, nsrc=37, seed=2, inside_cells=False)
+    elif name == "mixed":
+        w = mixed_workload(n=16, nsrc=150, seed=3)
+    elif name == "outside":
+        w = mixed_workload(n=16, nsrc=150, seed=6, lo=-1.5, hi=2.5)
     else:
         rng = np.random.default_rng(4)
         pos, _ = ic.clumped_sources(rng, 64, 300, nclump=3, sigma_cells=3.0)
