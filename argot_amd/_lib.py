@@ -39,6 +39,7 @@ EXPORTS = [
     "argot_hip_set_iterations", "input_mesh_single", "input_mesh", "input_src_file", "input_src",
     "input_data", "output_mesh_single", "output_mesh", "output_src", "output_data",
     "argot_hip_set_max_sources", "argot_hip_set_chem_queue", "argot_hip_set_wide_index",
+    "argot_hip_release_gpu",
 ]
 
 _lib = None
@@ -61,6 +62,8 @@ def lib():
     L.setup_cross_section.argtypes = [P(abi.CrossSection), P(abi.FreqParam)]
     cm, cp, rp = P(abi.CudaMemSpace), P(abi.CudaParam), P(abi.RunParam)
     L.init_gpu.argtypes = [C.c_void_p, cm, cp, rp]
+    L.argot_hip_release_gpu.argtypes = [cm]
+    L.argot_hip_release_gpu.restype = C.c_int
     L.send_mesh_data.argtypes = [C.c_void_p, cm, cp, rp]
     L.recv_mesh_data.argtypes = [C.c_void_p, cm, cp, rp]
     L.send_run_param_data.argtypes = [rp, cm, cp]
@@ -91,6 +94,9 @@ def lib():
     L.argot_hip_record_size.restype = C.c_int64
     L.argot_hip_abi_layout.argtypes = [P(C.c_int64), C.c_int]
     L.argot_hip_abi_layout.restype = C.c_int
+    L.argot_hip_segment_start.argtypes = [rp, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                          C.c_void_p]
+    L.argot_hip_segment_start.restype = C.c_int
     L.argot_hip_tree_build.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                        C.c_float]
     L.argot_hip_tree_build.restype = C.c_void_p

@@ -121,7 +121,9 @@ class StepStats(C.Structure):
                 ("nfallback", C.c_uint64), ("ms_build", C.c_double), ("ms_kernel", C.c_double),
                 ("ms_total", C.c_double), ("nsurrogate", C.c_uint64),
                 ("iterations", C.c_uint64), ("ms_chem", C.c_double),
-                ("nvisit", C.c_uint64), ("nemit_iter", C.c_uint64), ("ms_walk", C.c_double)]
+                ("nvisit", C.c_uint64), ("nemit_iter", C.c_uint64), ("ms_walk", C.c_double),
+                ("ms_rate", C.c_double), ("nspec_call", C.c_uint64),
+                ("noverflow", C.c_uint64)]
 
 
 class ChemStats(C.Structure):

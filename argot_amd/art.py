@@ -123,3 +123,5 @@ class DiffuseStep:
         L = lib()
         L.free_diffuse_gpu(self.cd, C.byref(self.cpar))
         L.free_hd_param(C.byref(self.hd))
+        if self.cmem.mesh_dev:
+            L.argot_hip_release_gpu(C.byref(self.cmem))

@@ -37,7 +37,7 @@ __device__ __forceinline__ float div_by(float a, double rb) {
 
 // Target cell centre, setup_light_ray_tree.c:341-343 / calc_ray_segment.c:48:
 // (float)i + 0.5 is double, the sum is rounded to float once.
-__device__ __forceinline__ float cell_centre(float lo, int i, float h) {
+ARGOT_HD inline float cell_centre(float lo, int i, float h) {
   return (float)((double)lo + ((double)(float)i + 0.5) * (double)h);
 }
 
@@ -49,8 +49,14 @@ struct Vec3f {
 // optical depth the ray keeps (a source outside the box enters at a face;
 // the domain loop can split that entry into several local segments, of
 // which accum_optical_depth.c:93-101 keeps the last).  Returns the number of
-// segments the reference would form (for the NSEG_PER_RAY check).
-__device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3f* start) {
+// segments the reference forms, counted up to kMaxSegments: the reference
+// asserts num_segment < NSEG_PER_RAY (calc_ray_segment.c:212), so a return
+// of kMaxSegments means the ray is refused (its start point is then the
+// one after kMaxSegments - 1 domain steps, not the reference's); on the
+// device such a ray also adds one to *refused when that is given.  Host and
+// device compile the same IEEE operations, so both give the same bits.
+ARGOT_HD inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3f* start,
+                                  unsigned long long* refused = nullptr) {
   const float dxd = (g.xmax - g.xmin) / 1.0f;
   const float dyd = (g.ymax - g.ymin) / 1.0f;
   const float dzd = (g.zmax - g.zmin) / 1.0f;
@@ -118,6 +124,9 @@ __device__ inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3
     sz = sz + rmin * zo;
     nseg++;
   }
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (refused && nseg >= kMaxSegments) atomicAdd(refused, 1ull);
+#endif
   start->x = sx;
   start->y = sy;
   start->z = sz;
@@ -267,15 +276,25 @@ __device__ __forceinline__ float4 cell_fields(const float4* __restrict__ nf, uin
     return nf[idx];
 }
 
-// The whole walk of one segment (calc_optical_depth.c:76-227).  nf holds the
-// per-cell FP32 densities (nHI, nHeI, nHeII) formed as the reference forms
-// them.  The next cell's densities are fetched before the current chord is
-// summed, so one gather is always in flight.  KAHAN selects compensated
-// sums; HASH records the cell sequence (FNV-1a over the linear cell
-// indices, the end cell last).
-template <bool KAHAN, bool HASH, bool NARROW>
-__device__ inline void walk_segment(const GridDev& g, const float4* __restrict__ nf, Vec3f s,
-                                    Vec3f e, WalkOut* out) {
+// Per-cell FP32 densities (nHI, nHeI, nHeII) of cell idx from the packed
+// float4 array.
+template <bool NARROW>
+struct PackedFields {
+  const float4* __restrict__ nf;
+  __device__ __forceinline__ float4 operator()(uint32_t idx) const {
+    return cell_fields<NARROW>(nf, idx);
+  }
+};
+
+// The whole walk of one segment (calc_optical_depth.c:76-227).  fields(idx)
+// gives the per-cell FP32 densities (nHI, nHeI, nHeII) formed as the
+// reference forms them.  The next cell's densities are fetched before the
+// current chord is summed, so one gather is always in flight.  KAHAN selects
+// compensated sums; HASH records the cell sequence (FNV-1a over the linear
+// cell indices, the end cell last).
+template <bool KAHAN, bool HASH, typename Fields>
+__device__ inline void walk_segment_with(const GridDev& g, const Fields& fields, Vec3f s, Vec3f e,
+                                         WalkOut* out) {
   Dda d;
   dda_setup(g, s, e, &d);
   float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
@@ -283,13 +302,13 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
   bool act = dda_active(d);
-  float4 n = cell_fields<NARROW>(nf, act ? d.idx : 0u);
+  float4 n = fields(act ? d.idx : 0u);
   while (act) {
     if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
     nc++;
     const float rmin = dda_step(&d);
     const bool nact = dda_active(d);
-    const float4 nn = cell_fields<NARROW>(nf, nact ? d.idx : 0u);
+    const float4 nn = fields(nact ? d.idx : 0u);
     if (KAHAN) {
       kahan_add(t0, c0, rmin * n.x);
       kahan_add(t1, c1, rmin * n.y);
@@ -304,7 +323,7 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   }
   const float depth = dda_depth(g, d, e);
   const uint32_t ie = d.idxe;
-  n = cell_fields<NARROW>(nf, ie);
+  n = fields(ie);
   if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
   nc++;
   if (KAHAN) {
@@ -321,6 +340,13 @@ __device__ inline void walk_segment(const GridDev& g, const float4* __restrict__
   out->tau[2] = t2;
   out->ncross = nc;
   out->hash = h;
+}
+
+// The walk over the packed per-cell densities nf (float4 per cell).
+template <bool KAHAN, bool HASH, bool NARROW>
+__device__ __forceinline__ void walk_segment(const GridDev& g, const float4* __restrict__ nf,
+                                             Vec3f s, Vec3f e, WalkOut* out) {
+  walk_segment_with<KAHAN, HASH>(g, PackedFields<NARROW>{nf}, s, e, out);
 }
 
 // delta_length, calc_photoion_rate.c:52-90.

@@ -81,6 +81,21 @@ struct RayRecord {
   double rate[6];       // this ray's rates (heats times hplanck*nuL)
 };
 
+// calc_photoion_rate_cuda inputs after the host's regrouping: one record
+// per ray (32 B) and one per target cell (32 B).
+struct CallRay {
+  float src[3];      // ray source position
+  float tau[3];      // optical depths (code units)
+  int32_t spec;      // index of the ray's spectrum among the call's distinct ones
+  int32_t pad;
+};
+struct CallTarget {
+  int64_t first;     // its first ray
+  int64_t cell;      // local linear cell index
+  int32_t count;     // its rays
+  int32_t g[3];      // global cell indices (target centre)
+};
+
 struct StepCounters {
   unsigned long long nray;
   unsigned long long ncross;
@@ -89,6 +104,7 @@ struct StepCounters {
   unsigned long long nsurrogate;
   unsigned long long nvisit;       // tree nodes visited, summed over wavefronts
   unsigned long long nemit_iter;   // of those, visits where some lane emitted a ray
+  unsigned long long noverflow;    // rays that would need NSEG_PER_RAY or more segments
   int queue[kNumQueues];
 };
 

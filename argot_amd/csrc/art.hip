@@ -728,6 +728,10 @@ void init_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
+  // The sizes are set only once every buffer exists, so that a failed
+  // allocation leaves a context that every later call refuses.
+  a_.ncell = 0;
+  a_.batch = 0;
   A_OK(hipMalloc(&a_.d_dirs, sizeof(ArtDir) * nang));
   A_OK(hipMalloc(&a_.med, sizeof(float4) * 2 * ncell));
   A_OK(hipMalloc(&a_.acc, sizeof(float4) * ncell * batch));
@@ -751,7 +755,7 @@ void send_diffuse_data(struct host_diffuse_param* hd, struct cuda_diffuse_param*
   (void)cd;
   (void)this_cuda;
   if (!hd || !hd->angle || !hd->step_fact) return argot_hip_internal_set_error(ARGOT_ERR_ARG);
-  if (!a_.d_dirs) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
+  if (!a_.d_dirs || a_.ncell <= 0) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
   const int nang = 12 * a_.nside * a_.nside;
   a_.dirs.resize(nang);
   for (int i = 0; i < nang; i++) {
@@ -782,7 +786,9 @@ void calc_diffuse_photon_radiation(struct fluid_mesh* mesh, struct run_param* th
   const double t_start = now_ms();
   if (!this_run || !cuda_mem || !cuda_mem->mesh_dev)
     return argot_hip_internal_set_error(ARGOT_ERR_STATE);
-  if (a_.dirs.empty() || !a_.acc) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
+  if (a_.dirs.empty() || !a_.d_dirs || !a_.med || !a_.acc || !a_.medT || !a_.gh || !a_.comp ||
+      !a_.ncross || a_.ncell <= 0 || a_.batch <= 0)
+    return argot_hip_internal_set_error(ARGOT_ERR_STATE);
   ArtGrid g;
   g.nx = this_run->nmesh_x_local;
   g.ny = this_run->nmesh_y_local;
@@ -886,6 +892,8 @@ void free_diffuse_gpu(struct cuda_diffuse_param* cd, struct cuda_param* this_cud
   a_.gh = a_.comp = nullptr;
   a_.ncross = nullptr;
   a_.dirs.clear();
+  a_.ncell = 0;
+  a_.batch = 0;
   if (cd) cd[0].rmesh = nullptr;
 }
 

@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "argot_hip.h"
@@ -43,6 +44,19 @@ struct mesh_io {
 static_assert(sizeof(mesh_io) == 80, "struct fluid_mesh_io is 80 bytes");
 
 void fail(int code) { argot::argot_hip_internal_set_error(code); }
+
+// Runs an entry point's body so that no exception crosses the C-ABI: a
+// failed allocation is ARGOT_ERR_NOMEM, anything else ARGOT_ERR_ARG.
+template <typename F>
+void guarded(F&& body) {
+  try {
+    body();
+  } catch (const std::bad_alloc&) {
+    fail(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    fail(ARGOT_ERR_ARG);
+  }
+}
 
 struct File {
   FILE* f = nullptr;
@@ -98,8 +112,8 @@ extern "C" {
 
 void argot_hip_set_max_sources(uint64_t n) { g_max_sources = n; }
 
-void input_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char* filename) {
-  argot::argot_hip_internal_clear_error();
+static void input_mesh_single_body(struct fluid_mesh* mesh, struct run_param* this_run,
+                                   char* filename) {
   if (!mesh || !this_run || !filename) return fail(ARGOT_ERR_ARG);
   const int64_t n = local_cells(this_run);
   if (n < 0) return fail(ARGOT_ERR_ARG);
@@ -131,6 +145,11 @@ void input_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char
   memcpy(this_run, &h, sizeof h);
 }
 
+void input_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char* filename) {
+  argot::argot_hip_internal_clear_error();
+  guarded([&] { input_mesh_single_body(mesh, this_run, filename); });
+}
+
 void input_mesh(struct fluid_mesh* mesh, struct run_param* this_run, char* prefix) {
   if (!this_run || !prefix) {
     argot::argot_hip_internal_clear_error();
@@ -141,8 +160,8 @@ void input_mesh(struct fluid_mesh* mesh, struct run_param* this_run, char* prefi
   input_mesh_single(mesh, this_run, name);
 }
 
-void input_src_file(struct radiation_src* src, struct run_param* this_run, char* src_file) {
-  argot::argot_hip_internal_clear_error();
+static void input_src_file_body(struct radiation_src* src, struct run_param* this_run,
+                                char* src_file) {
   if (!src || !this_run || !src_file) return fail(ARGOT_ERR_ARG);
   File in(src_file, "rb");
   if (!in.f) return fail(ARGOT_ERR_IO);
@@ -151,11 +170,19 @@ void input_src_file(struct radiation_src* src, struct run_param* this_run, char*
   if (nsrc > g_max_sources) return fail(ARGOT_ERR_ARG);
   freq_param freq;
   if (!in.read(&freq, sizeof freq)) return fail(ARGOT_ERR_IO);
+  // a short file is refused before the records are allocated
+  const int64_t left = in.remaining();
+  if (left < 0 || (uint64_t)left / sizeof(radiation_src) < nsrc) return fail(ARGOT_ERR_IO);
   std::vector<radiation_src> s((size_t)nsrc);
   if (!in.read(s.data(), sizeof(radiation_src) * s.size())) return fail(ARGOT_ERR_IO);
   this_run->nsrc = nsrc;
   this_run->freq = freq;
   if (nsrc) memcpy(src, s.data(), sizeof(radiation_src) * s.size());
+}
+
+void input_src_file(struct radiation_src* src, struct run_param* this_run, char* src_file) {
+  argot::argot_hip_internal_clear_error();
+  guarded([&] { input_src_file_body(src, this_run, src_file); });
 }
 
 void input_src(struct radiation_src* src, struct run_param* this_run, char* prefix) {
@@ -175,8 +202,8 @@ void input_data(struct fluid_mesh* mesh, struct radiation_src* src, struct run_p
   input_src(src, this_run, prefix);
 }
 
-void output_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char* filename) {
-  argot::argot_hip_internal_clear_error();
+static void output_mesh_single_body(struct fluid_mesh* mesh, struct run_param* this_run,
+                                    char* filename) {
   if (!mesh || !this_run || !filename) return fail(ARGOT_ERR_ARG);
   const int64_t n = local_cells(this_run);
   if (n < 0) return fail(ARGOT_ERR_ARG);
@@ -201,6 +228,11 @@ void output_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, cha
         !out.write(rec.data(), sizeof(mesh_io) * (size_t)(j + 1)))
       return fail(ARGOT_ERR_IO);
   }
+}
+
+void output_mesh_single(struct fluid_mesh* mesh, struct run_param* this_run, char* filename) {
+  argot::argot_hip_internal_clear_error();
+  guarded([&] { output_mesh_single_body(mesh, this_run, filename); });
 }
 
 void output_mesh(struct fluid_mesh* mesh, struct run_param* this_run, char* prefix) {

@@ -31,10 +31,14 @@ hipError_t launch_set_rates(void* mesh, const float* rate, int64_t ncell, hipStr
 // than 2^28 cells (meshes with more always are).
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
                                 int kahan, int wide, hipStream_t s);
-hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* mesh, const void* rays, int64_t nray, double* per_ray,
-                            int64_t* target, hipStream_t s);
-hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
-                                  void* mesh, hipStream_t s);
+// calc_photoion_rate_cuda: ntarget targets, each with its consecutive rays
+// (rays[first .. first + count)), rates added to the mesh's prev_chem.
+// spec32/spec64/spec_scale: the distinct spectra ([nspec * ngrid] FP32
+// scaled by 1/spec_scale, FP64 as given, and the per-spectrum maximum).
+hipError_t launch_target_rates(const GridDev& g, const RateTables& rt, void* mesh,
+                              const CallTarget* tg, int64_t ntarget, const CallRay* rays,
+                              const float* spec32, const double* spec64, const double* spec_scale,
+                              hipStream_t s);
 hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s);
 // smooth_photoion_rate on one rank; tmp_in/tmp_out: 6*ncell floats each.
 // On return (stream order) tmp_out holds the smoothed rates as six planes.

@@ -4,8 +4,7 @@
 //   zero_rates          prev_chem rates := 0         (zero_out_photoion_rate)
 //   add_rates           prev_chem rates += table     (merge_photoion_data)
 //   walk_segments       optical depth of a segment list (calc_optical_depth)
-//   ray_rates           per-ray rates of a light_ray_IO list (calc_photoion_rate)
-//   merge_ray_rates     per-target sums of consecutive rays, no atomics
+//   target_rates        per-target rates and sums of a ray list (calc_photoion_rate)
 //   fused_step          tree walk + segment + optical depth + rates + per-cell
 //                       sum for 4x4x4 target tiles, one wavefront per tile
 #include <hip/hip_runtime.h>
@@ -89,11 +88,12 @@ __global__ void set_rates_kernel(unsigned char* __restrict__ mesh, const float* 
 // estimated crossing count (counting sort in LDS), so that the lanes of a
 // wavefront walk segments of similar length; each segment's result does not
 // depend on which lane walks it.
+// Fields: where the per-cell densities come from (PackedFields: the packed
+// float4 array).
 constexpr int kWalkBlock = 256;
-template <bool NARROW>
+template <typename Fields>
 __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
-    GridDev g, const float4* __restrict__ nf, unsigned char* __restrict__ seg, int64_t nseg,
-    int kahan) {
+    GridDev g, Fields fields, unsigned char* __restrict__ seg, int64_t nseg, int kahan) {
   __shared__ unsigned bucket_lds[64];
   __shared__ unsigned perm_lds[kWalkBlock];
   const int t = threadIdx.x;
@@ -129,9 +129,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
   const Vec3f e{rec[7], rec[8], rec[9]};
   WalkOut w;
   if (kahan)
-    walk_segment<true, false, NARROW>(g, nf, s, e, &w);
+    walk_segment_with<true, false>(g, fields, s, e, &w);
   else
-    walk_segment<false, false, NARROW>(g, nf, s, e, &w);
+    walk_segment_with<false, false>(g, fields, s, e, &w);
   rec[10] = w.tau[0];
   rec[11] = w.tau[1];
   rec[12] = w.tau[2];
@@ -298,60 +298,6 @@ __device__ __forceinline__ bool fast_rate_ok(const RateTables& rt, const RayRate
   return finite && lo > -60.0 && dmax < 1.0e30;
 }
 
-// Legacy per-ray rate kernel: one thread per light_ray_IO record (2096 B:
-// src@0 {type, param, xpos@8, ypos, zpos, photon_rate@24}, ix_target@2072,
-// tau@2084).  Per-ray results go to out[6*i] for the segmented merge.
-__global__ void ray_rates_kernel(GridDev g, RateTables rt, const unsigned char* __restrict__ mesh,
-                                 const unsigned char* __restrict__ rays, int64_t nray,
-                                 double* __restrict__ out, int64_t* __restrict__ target) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nray) return;
-  const unsigned char* r = rays + i * 2096;
-  const float* rf = reinterpret_cast<const float*>(r);
-  const double* spec = reinterpret_cast<const double*>(r + 24);
-  const int* it = reinterpret_cast<const int*>(r + 2072);
-  const Vec3f src{rf[2], rf[3], rf[4]};
-  // global target indices (calc_photoion_rate.c:176-184): local cell for
-  // the mesh, global origin for the position
-  const int ix = it[0], iy = it[1], iz = it[2];
-  const float* tau = reinterpret_cast<const float*>(r + 2084);
-  const int lx = ix - g.ox, ly = iy - g.oy, lz = iz - g.oz;
-  const int64_t c = (int64_t)lz + (int64_t)g.nz * ((int64_t)ly + (int64_t)g.ny * lx);
-  target[i] = c;
-  const Vec3f tgt{cell_centre(g.gxmin, ix, g.dx), cell_centre(g.gymin, iy, g.dy),
-                  cell_centre(g.gzmin, iz, g.dz)};
-  RayRate rr;
-  ray_rate_setup(g, rt, tau, src, tgt, *mesh_f(mesh, c, kOffDens), *mesh_f(mesh, c, kOffFHI),
-                 *mesh_f(mesh, c, kOffFHeI), *mesh_f(mesh, c, kOffFHeII), &rr);
-  double res[6];
-  if (fast_rate_ok(rt, rr)) {
-    double smax = 0.0;
-    for (int k = 0; k < rt.ngrid; k++) smax = fmax(smax, spec[k]);
-    const double inv = smax > 0.0 ? 1.0 / smax : 0.0;
-    rate_loop_f32(rt, rr, smax, [&](int k) { return (float)(spec[k] * inv); }, res);
-  } else {
-    rate_loop_f64(rt, rr, [&](int k) { return spec[k]; }, res);
-  }
-  for (int k = 0; k < 6; k++) out[6 * i + k] = res[k];
-}
-
-// Rays of one target are consecutive (setup_light_ray_range emits them in
-// target order): the thread owning the first ray of each run sums the run
-// in ray order in FP64 and adds it to the cell once.
-__global__ void merge_ray_rates_kernel(const double* __restrict__ per_ray,
-                                       const int64_t* __restrict__ target, int64_t nray,
-                                       unsigned char* __restrict__ mesh) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nray) return;
-  const int64_t c = target[i];
-  if (i > 0 && target[i - 1] == c) return;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t j = i; j < nray && target[j] == c; j++)
-    for (int k = 0; k < 6; k++) acc[k] += per_ray[6 * j + k];
-  float* r = mesh_fw(mesh, c, kOffRates);
-  for (int k = 0; k < 6; k++) r[k] = (float)((double)r[k] + acc[k]);
-}
-
 __device__ __forceinline__ int wave_min(int v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
@@ -363,6 +309,79 @@ __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// ARGOT_WAVES_PER_EU (build option) caps the register budget so that this
+// many wavefronts fit per SIMD.
+#if defined(ARGOT_WAVES_PER_EU) && ARGOT_WAVES_PER_EU > 0
+#define ARGOT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARGOT_WAVES_PER_EU)))
+#else
+#define ARGOT_OCCUPANCY
+#endif
+#define ARGOT_CALL_OCCUPANCY ARGOT_OCCUPANCY
+
+// calc_photoion_rate_cuda (calc_photoion_rate_cuda.cu:92-216 with the
+// per-target merge of calc_photoion_rate.c:403-425): one lane per target
+// cell.  A target's rays are consecutive in the caller's order and its lane
+// sums their rates in FP64 in that order, then adds the sum to the cell
+// once: no atomics, no per-ray buffer.  Each ray names one of the call's
+// distinct spectra (rays from one source or tree node share it).  The
+// wavefront visits the spectra its lanes' next rays use in increasing index
+// order (the host numbers them by first appearance, so the lists of
+// neighbouring targets run nearly in step): the lanes at that spectrum take
+// their ray with the bins read through scalar loads, the others wait.  Each
+// ray's arithmetic is the fused kernel's, so a ray list taken from the
+// fused step gives its bits.
+__global__ __launch_bounds__(64) ARGOT_CALL_OCCUPANCY void target_rates_kernel(
+    GridDev g, RateTables rt, unsigned char* __restrict__ mesh, const CallTarget* __restrict__ tg,
+    int64_t ntarget, const CallRay* __restrict__ rays, const float* __restrict__ spec32,
+    const double* __restrict__ spec64, const double* __restrict__ spec_scale) {
+  const int lane = threadIdx.x;
+  const int64_t t = (int64_t)blockIdx.x * 64 + lane;
+  const bool valid = t < ntarget;
+  CallTarget T{0, 0, 0, {0, 0, 0}};
+  if (valid) T = tg[t];
+  float dens = 0.0f, fHI = 0.0f, fHeI = 0.0f, fHeII = 0.0f;
+  if (valid) {
+    dens = *mesh_f(mesh, T.cell, kOffDens);
+    fHI = *mesh_f(mesh, T.cell, kOffFHI);
+    fHeI = *mesh_f(mesh, T.cell, kOffFHeI);
+    fHeII = *mesh_f(mesh, T.cell, kOffFHeII);
+  }
+  const Vec3f tgt{cell_centre(g.gxmin, T.g[0], g.dx), cell_centre(g.gymin, T.g[1], g.dy),
+                  cell_centre(g.gzmin, T.g[2], g.dz)};
+  int64_t r = T.first;
+  const int64_t rend = T.first + T.count;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int ng = rt.ngrid;
+  for (;;) {
+    const bool more = r < rend;
+    const int sp = more ? rays[r].spec : INT_MAX;
+    const int u = wave_min(sp);
+    if (u == INT_MAX) break;
+    if (more && sp == u) {
+      const CallRay R = rays[r];
+      const Vec3f src{R.src[0], R.src[1], R.src[2]};
+      RayRate rr;
+      ray_rate_setup(g, rt, R.tau, src, tgt, dens, fHI, fHeI, fHeII, &rr);
+      double res[6];
+      if (fast_rate_ok(rt, rr)) {
+        const float* s32 = spec32 + (int64_t)u * ng;
+        rate_loop_f32(rt, rr, ldc(spec_scale + u), [&](int k) { return ldc(s32 + k); }, res);
+      } else {
+        const double* s64 = spec64 + (int64_t)u * ng;
+        rate_loop_f64(rt, rr, [&](int k) { return ldc(s64 + k); }, res);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) acc[k] += res[k];
+      r++;
+    }
+  }
+  if (valid && T.count > 0) {
+    float* o = mesh_fw(mesh, T.cell, kOffRates);
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = (float)((double)o[k] + acc[k]);
+  }
 }
 
 // Next tile for this wavefront: its own XCD's queue first (blocks are
@@ -381,13 +400,6 @@ __device__ __forceinline__ int64_t next_tile(StepCounters* ctr, int64_t ntile, i
   return -1;
 }
 
-// ARGOT_WAVES_PER_EU (build option) caps the register budget so that this
-// many wavefronts fit per SIMD.
-#if defined(ARGOT_WAVES_PER_EU) && ARGOT_WAVES_PER_EU > 0
-#define ARGOT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARGOT_WAVES_PER_EU)))
-#else
-#define ARGOT_OCCUPANCY
-#endif
 
 // 2-point Gauss-Legendre nodes on [0, 1].
 constexpr double kGl1 = 0.21132486540518711775, kGl2 = 0.78867513459481288225;
@@ -524,7 +536,9 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
           src = Vec3f{tr.src_pos[4 * ln.src + 0], tr.src_pos[4 * ln.src + 1],
                       tr.src_pos[4 * ln.src + 2]};
         }
-        segment_start(g, src, tgt, &st);
+        // a ray the reference refuses (calc_ray_segment.c:212) is counted
+        // where it happens, on the path of sources outside the box
+        segment_start(g, src, tgt, &st, &a.ctr->noverflow);
         walk_segment<KAHAN, CAPTURE, NARROW>(g, a.nf, st, tgt, &w);
         const float4 f = cell_lds[lane];
         ray_rate_setup(g, rt, w.tau, src, tgt, f.x, f.y, f.z, f.w, &rr);
@@ -699,6 +713,7 @@ __global__ void reset_counters_kernel(StepCounters* ctr) {
     ctr->nsurrogate = 0;
     ctr->nvisit = 0;
     ctr->nemit_iter = 0;
+    ctr->noverflow = 0;
   }
 }
 
@@ -729,28 +744,21 @@ hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, i
                                 int kahan, int wide, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
   if (!wide && (int64_t)g.nx * g.ny * g.nz < kNarrowCells)
-    walk_segments_kernel<true><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-        g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
+    walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, PackedFields<true>{nf}, static_cast<unsigned char*>(seg), nseg, kahan);
   else
-    walk_segments_kernel<false><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-        g, nf, static_cast<unsigned char*>(seg), nseg, kahan);
+    walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, PackedFields<false>{nf}, static_cast<unsigned char*>(seg), nseg, kahan);
   return hipGetLastError();
 }
 
-hipError_t launch_ray_rates(const GridDev& g, const RateTables& rt, const void* mesh, const void* rays, int64_t nray, double* per_ray,
-                            int64_t* target, hipStream_t s) {
-  if (nray <= 0) return hipSuccess;
-  ray_rates_kernel<<<nblocks(nray, 128), 128, 0, s>>>(
-      g, rt, static_cast<const unsigned char*>(mesh),
-      static_cast<const unsigned char*>(rays), nray, per_ray, target);
-  return hipGetLastError();
-}
-
-hipError_t launch_merge_ray_rates(const double* per_ray, const int64_t* target, int64_t nray,
-                                  void* mesh, hipStream_t s) {
-  if (nray <= 0) return hipSuccess;
-  merge_ray_rates_kernel<<<nblocks(nray, 256), 256, 0, s>>>(per_ray, target, nray,
-                                                             static_cast<unsigned char*>(mesh));
+hipError_t launch_target_rates(const GridDev& g, const RateTables& rt, void* mesh,
+                              const CallTarget* tg, int64_t ntarget, const CallRay* rays,
+                              const float* spec32, const double* spec64, const double* spec_scale,
+                              hipStream_t s) {
+  if (ntarget <= 0) return hipSuccess;
+  target_rates_kernel<<<nblocks(ntarget, 64), 64, 0, s>>>(
+      g, rt, static_cast<unsigned char*>(mesh), tg, ntarget, rays, spec32, spec64, spec_scale);
   return hipGetLastError();
 }
 

@@ -10,9 +10,12 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <unordered_map>
 #include <vector>
 
+#include "argot_device.h"
 #include "argot_hip.h"
 #include "argot_internal.h"
 #include "kernels.h"
@@ -74,6 +77,11 @@ template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t cap = 0;  // elements
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
   hipError_t reserve(size_t n) {
     if (n <= cap && p) return hipSuccess;
     if (p) {
@@ -89,6 +97,11 @@ struct DevBuf {
   }
 };
 
+// Everything one mesh needs on the device: its own stream, events, field
+// and tree buffers, staging and the captured step graph.  init_gpu creates
+// one; the caller's cuda_mem_space (its mesh_dev) names it in every later
+// call, so meshes and streams of one process do not share state and no
+// entry point depends on the calling thread's state.
 struct Ctx {
   bool ready = false;
   hipStream_t stream = nullptr;
@@ -111,8 +124,6 @@ struct Ctx {
   DevBuf<RayRecord> rec;
   // segment / ray staging for the per-call entry points
   DevBuf<unsigned char> seg, rays;
-  DevBuf<double> per_ray;
-  DevBuf<int64_t> target;
   DevBuf<float> rate_tab;
   DevBuf<float> smooth_in, smooth_out;   // smooth_photoion_rate planes
   DevBuf<ChemCounters> chem_ctr;
@@ -125,9 +136,72 @@ struct Ctx {
   hipGraphExec_t graph_exec = nullptr;
   GraphKey graph_key{};
   int64_t stats_graph_captures = 0;
+  // the caller-visible device buffers init_gpu allocated
+  void* run_dev = nullptr;
+  void* rate_dev = nullptr;
+
+  ~Ctx() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamSynchronize(stream);
+    nf.release();
+    nu.release();
+    cs.release();
+    sig.release();
+    hsig.release();
+    dsig.release();
+    link.release();
+    cm.release();
+    spec.release();
+    src_pos.release();
+    spec_scale.release();
+    ctr.release();
+    node_q.release();
+    rec.release();
+    seg.release();
+    rays.release();
+    rate_tab.release();
+    smooth_in.release();
+    smooth_out.release();
+    chem_ctr.release();
+    for (hipEvent_t e : {ev0, ev1, ev_mid, ev_c0, ev_c1})
+      if (e) (void)hipEventDestroy(e);
+    if (mesh) (void)hipFree(mesh);
+    if (run_dev) (void)hipFree(run_dev);
+    if (rate_dev) (void)hipFree(rate_dev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
 };
 
-Ctx g;
+// Live contexts, looked up by the device mesh their init_gpu allocated.
+std::mutex g_ctx_mu;
+std::vector<std::unique_ptr<Ctx>> g_ctx;
+// The context of this thread's last call (for the statistics queries).
+thread_local Ctx* t_last = nullptr;
+
+Ctx* find_ctx(const cuda_mem_space* cm) {
+  if (!cm || !cm->mesh_dev) return nullptr;
+  const unsigned char* key = reinterpret_cast<const unsigned char*>(cm->mesh_dev);
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  for (auto& c : g_ctx)
+    if (c->mesh == key) return c.get();
+  return nullptr;
+}
+
+// Removes and frees the context whose device mesh is mesh_dev, if any.
+void drop_ctx(const void* mesh_dev) {
+  std::unique_ptr<Ctx> gone;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    for (auto it = g_ctx.begin(); it != g_ctx.end(); ++it)
+      if ((*it)->mesh == mesh_dev) {
+        gone = std::move(*it);
+        g_ctx.erase(it);
+        break;
+      }
+  }
+  if (gone && t_last == gone.get()) t_last = nullptr;
+}
 
 GridDev make_grid(const run_param* r) {
   GridDev d{};   // zeroed padding: launch arguments are compared as bytes
@@ -158,7 +232,7 @@ GridDev make_grid(const run_param* r) {
   return d;
 }
 
-RateTables rate_tables() {
+RateTables rate_tables(const Ctx& g) {
   RateTables t{};
   t.ngrid = g.ngrid;
   t.nu = g.nu.p;
@@ -176,7 +250,7 @@ RateTables rate_tables() {
 }
 
 // Tables of run_param (freq.nu, csect) in the layouts the kernels read.
-void upload_tables(const run_param* r) {
+void upload_tables(Ctx& g, const run_param* r) {
   const int ng = ARGOT_NGRID_NU;
   std::vector<float> nu(ng), cs(3 * ng), sig(3 * ng), hsig(3 * ng), dsig(3 * ng);
   const double thr[3] = {1.0, 1.8125, 4.0};   // HI, HeI, HeII limits [nu_L]
@@ -219,20 +293,23 @@ void upload_tables(const run_param* r) {
   HIP_OK(hipMemcpy(g.dsig.p, dsig.data(), 3 * ng * sizeof(float), hipMemcpyHostToDevice));
 }
 
-bool check_ready(const cuda_mem_space* cm, const run_param* r) {
-  if (!g.ready || !cm || !cm->mesh_dev || !r) {
+// The caller's context, ready and for a mesh of this_run's size, or null
+// (with the error set).
+Ctx* check_ready(const cuda_mem_space* cm, const run_param* r) {
+  Ctx* c = find_ctx(cm);
+  if (!c || !c->ready || !r) {
     set_err(ARGOT_ERR_STATE);
-    return false;
+    return nullptr;
   }
-  if (r->nmesh_x_local != g.nx || r->nmesh_y_local != g.ny || r->nmesh_z_local != g.nz) {
+  t_last = c;
+  if (r->nmesh_x_local != c->nx || r->nmesh_y_local != c->ny || r->nmesh_z_local != c->nz) {
     set_err(ARGOT_ERR_ARG);
-    return false;
+    return nullptr;
   }
-  g.mesh = reinterpret_cast<unsigned char*>(cm->mesh_dev);
-  return true;
+  return c;
 }
 
-void pack_fields() {
+void pack_fields(Ctx& g) {
   HIP_OK(g.nf.reserve(g.ncell));
   HIP_OK(launch_pack_fields(g.mesh, g.ncell, g_helium_fact, g.nf.p, g.stream));
 }
@@ -273,8 +350,9 @@ const char* argot_hip_error_string(int code) {
 void argot_hip_set_helium_fact(double helium_fact) { g_helium_fact = helium_fact; }
 void argot_hip_set_tau_mode(int mode) { g_tau_mode = mode ? 1 : 0; }
 void argot_hip_set_rate_mode(int mode) { g_rate_mode = mode ? 1 : 0; }
+// Statistics of the last call on this thread (its context's).
 void argot_hip_step_stats(struct argot_step_stats* out) {
-  if (out) *out = g.stats;
+  if (out) *out = t_last ? t_last->stats : argot_step_stats{};
 }
 
 // Opening angle used by step_radiation_tree (the reference always sets
@@ -299,7 +377,7 @@ void argot_hip_set_tile(int tx, int ty, int tz) {
 
 void argot_hip_set_graph(int on) { g_use_graph = on ? 1 : 0; }
 
-int64_t argot_hip_graph_captures(void) { return g.stats_graph_captures; }
+int64_t argot_hip_graph_captures(void) { return t_last ? t_last->stats_graph_captures : 0; }
 
 // Test support: the next step_radiation_tree records every ray it forms
 // (struct RayRecord, 120 B) into host_records[0..cap).
@@ -330,6 +408,11 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   HIP_OK(hipSetDevice(0));
   hipDeviceProp_t prop;
   HIP_OK(hipGetDeviceProperties(&prop, 0));
+  // A cuda_mem that already names a context is being initialised again:
+  // that context (and its device mesh) goes.
+  if (find_ctx(cuda_mem)) drop_ctx(cuda_mem->mesh_dev);
+  std::unique_ptr<Ctx> owned(new Ctx());
+  Ctx& g = *owned;
   g.ncu = prop.multiProcessorCount;
   g.nx = this_run->nmesh_x_local;
   g.ny = this_run->nmesh_y_local;
@@ -342,32 +425,54 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
   this_cuda->max_thread_dimx = prop.maxThreadsDim[0];
   this_cuda->max_block_dimx = prop.maxGridSize[0];
   memset(cuda_mem, 0, sizeof(*cuda_mem));
-  HIP_OK(hipMalloc(&cuda_mem->mesh_dev, sizeof(struct fluid_mesh) * g.ncell));
-  HIP_OK(hipMalloc(&cuda_mem->this_run_dev, sizeof(struct run_param)));
-  HIP_OK(hipMalloc(&cuda_mem->rate_dev, sizeof(struct photoion_rate) * g.ncell));
-  HIP_OK(hipMemcpy(cuda_mem->this_run_dev, this_run, sizeof(struct run_param),
-                   hipMemcpyHostToDevice));
-  if (!g.stream) HIP_OK(hipStreamCreate(&g.stream));
-  if (!g.ev0) HIP_OK(hipEventCreate(&g.ev0));
-  if (!g.ev1) HIP_OK(hipEventCreate(&g.ev1));
-  if (!g.ev_mid) HIP_OK(hipEventCreate(&g.ev_mid));
-  if (!g.ev_c0) HIP_OK(hipEventCreate(&g.ev_c0));
-  if (!g.ev_c1) HIP_OK(hipEventCreate(&g.ev_c1));
-  this_cuda->strm[0] = g.stream;
-  this_cuda->diffuse_strm[0] = g.stream;
-  g.mesh = reinterpret_cast<unsigned char*>(cuda_mem->mesh_dev);
-  upload_tables(this_run);
+  // (owned frees what is allocated here if a later step fails)
+  HIP_OK(hipMalloc(&g.mesh, sizeof(struct fluid_mesh) * g.ncell));
+  HIP_OK(hipMalloc(&g.run_dev, sizeof(struct run_param)));
+  HIP_OK(hipMalloc(&g.rate_dev, sizeof(struct photoion_rate) * g.ncell));
+  HIP_OK(hipMemcpy(g.run_dev, this_run, sizeof(struct run_param), hipMemcpyHostToDevice));
+  HIP_OK(hipStreamCreate(&g.stream));
+  HIP_OK(hipEventCreate(&g.ev0));
+  HIP_OK(hipEventCreate(&g.ev1));
+  HIP_OK(hipEventCreate(&g.ev_mid));
+  HIP_OK(hipEventCreate(&g.ev_c0));
+  HIP_OK(hipEventCreate(&g.ev_c1));
+  upload_tables(g, this_run);
   if (t_err) return;
   HIP_OK(g.ctr.reserve(1));
   HIP_OK(g.chem_ctr.reserve(1));
   g.ready = true;
+  cuda_mem->mesh_dev = reinterpret_cast<struct fluid_mesh*>(g.mesh);
+  cuda_mem->this_run_dev = static_cast<struct run_param*>(g.run_dev);
+  cuda_mem->rate_dev = static_cast<struct photoion_rate*>(g.rate_dev);
+  this_cuda->strm[0] = g.stream;
+  this_cuda->diffuse_strm[0] = g.stream;
+  {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    g_ctx.push_back(std::move(owned));
+  }
+  t_last = &g;
+}
+
+// Frees the context init_gpu made for cuda_mem: its device mesh, run_param
+// and rate table, stream, events and buffers; cuda_mem is zeroed.  Returns
+// the status instead of setting the thread's last error, so that a release
+// run from a destructor cannot hide another call's error.
+int argot_hip_release_gpu(struct cuda_mem_space* cuda_mem) {
+  if (!cuda_mem) return ARGOT_ERR_ARG;
+  if (!find_ctx(cuda_mem)) return ARGOT_ERR_STATE;
+  drop_ctx(cuda_mem->mesh_dev);
+  memset(cuda_mem, 0, sizeof(*cuda_mem));
+  return ARGOT_OK;
 }
 
 void send_mesh_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
                     struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  if (!mesh) return set_err(ARGOT_ERR_ARG);
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   HIP_OK(hipMemcpyAsync(cuda_mem->mesh_dev, mesh, sizeof(struct fluid_mesh) * g.ncell,
                         hipMemcpyHostToDevice, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
@@ -377,7 +482,10 @@ void recv_mesh_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
                     struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  if (!mesh) return set_err(ARGOT_ERR_ARG);
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   HIP_OK(hipMemcpyAsync(mesh, cuda_mem->mesh_dev, sizeof(struct fluid_mesh) * g.ncell,
                         hipMemcpyDeviceToHost, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
@@ -387,10 +495,12 @@ void send_run_param_data(struct run_param* this_run, struct cuda_mem_space* cuda
                          struct cuda_param* this_cuda) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!g.ready || !cuda_mem || !this_run) return set_err(ARGOT_ERR_STATE);
-  HIP_OK(hipMemcpy(cuda_mem->this_run_dev, this_run, sizeof(struct run_param),
-                   hipMemcpyHostToDevice));
-  upload_tables(this_run);
+  Ctx* const cp = find_ctx(cuda_mem);
+  if (!cp || !cp->ready || !this_run) return set_err(ARGOT_ERR_STATE);
+  Ctx& g = *cp;
+  t_last = cp;
+  HIP_OK(hipMemcpy(g.run_dev, this_run, sizeof(struct run_param), hipMemcpyHostToDevice));
+  upload_tables(g, this_run);
 }
 
 void allocate_pinned_segment(struct ray_segment** seg, uint64_t nseg) {
@@ -423,11 +533,13 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
                              struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   const int64_t nseg = (int64_t)this_run->nseg;
   if (nseg <= 0) return;
   if (!seg) return set_err(ARGOT_ERR_ARG);
-  pack_fields();
+  pack_fields(g);
   if (t_err) return;
   const size_t bytes = sizeof(struct ray_segment) * nseg;
   HIP_OK(g.seg.reserve(bytes));
@@ -443,19 +555,40 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
   g.stats.ms_walk = ms;
 }
 
+// Distinct spectra of a ray list: rays from one source or tree node carry
+// equal photon_rate rows.  Candidates are found by three bins of the row and
+// confirmed by comparing the whole row, so only equal rows share an index;
+// indices follow the order of first appearance.
+struct SpectrumKey {
+  uint64_t a, b, c;
+  bool operator==(const SpectrumKey& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct SpectrumKeyHash {
+  size_t operator()(const SpectrumKey& k) const {
+    uint64_t h = k.a * 0x9e3779b97f4a7c15ull;
+    h ^= k.b + 0x7f4a7c159e3779b9ull + (h << 6) + (h >> 2);
+    h ^= k.c + 0x94d049bb133111ebull + (h << 6) + (h >> 2);
+    return (size_t)h;
+  }
+};
+
 static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
                                          struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
                                          struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   const int64_t nray = (int64_t)this_run->nray;
   if (nray <= 0) return;
   if (!ray || !ray_IO) return set_err(ARGOT_ERR_ARG);
+  if (nray > (int64_t)INT32_MAX) return set_err(ARGOT_ERR_ARG);
   const GridDev gd = make_grid(this_run);
-  // Repack light_ray -> light_ray_IO (calc_photoion_rate_cuda.cu:108-132),
-  // keeping each target's rays together in their original order so the
-  // per-target sums need no atomics.
+  const int ng = ARGOT_NGRID_NU;
+  // Each target's rays, together and in the caller's order
+  // (calc_photoion_rate_cuda.cu:108-132 repacks them; the per-target sums
+  // then need no atomics).
   std::vector<int64_t> tgt(nray);
   for (int64_t i = 0; i < nray; i++) {
     const int64_t lx = ray[i].ix_target - gd.ox, ly = ray[i].iy_target - gd.oy,
@@ -480,26 +613,98 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
     std::stable_sort(order.begin(), order.end(),
                      [&](int64_t a, int64_t b) { return tgt[a] < tgt[b]; });
   }
+  // Staging in the caller's pinned ray_IO buffer (nray * 2096 bytes): the
+  // ray records, then the target records, then the spectrum tables when
+  // they fit; the tables go from host memory otherwise.
+  const size_t cap = sizeof(struct light_ray_IO) * (size_t)nray;
+  unsigned char* stage = reinterpret_cast<unsigned char*>(ray_IO);
+  CallRay* cr = reinterpret_cast<CallRay*>(stage);
+  CallTarget* ct = reinterpret_cast<CallTarget*>(stage + sizeof(CallRay) * nray);
+  int64_t ntarget = 0;
+  std::unordered_map<SpectrumKey, std::vector<int32_t>, SpectrumKeyHash> index;
+  std::vector<const double*> rows;   // first ray's row of each distinct spectrum
   for (int64_t j = 0; j < nray; j++) {
     const int64_t i = contiguous ? j : order[j];
-    light_ray_IO& o = ray_IO[j];
-    o.src = ray[i].src;
-    o.ix_target = ray[i].ix_target;
-    o.iy_target = ray[i].iy_target;
-    o.iz_target = ray[i].iz_target;
-    o.optical_depth_HI = ray[i].optical_depth_HI;
-    o.optical_depth_HeI = ray[i].optical_depth_HeI;
-    o.optical_depth_HeII = ray[i].optical_depth_HeII;
+    const light_ray& R = ray[i];
+    const double* row = R.src.photon_rate;
+    SpectrumKey key;
+    memcpy(&key.a, &row[0], 8);
+    memcpy(&key.b, &row[ng / 2], 8);
+    memcpy(&key.c, &row[ng - 1], 8);
+    std::vector<int32_t>& cand = index[key];
+    int32_t sp = -1;
+    for (int32_t c : cand)
+      if (memcmp(rows[c], row, sizeof(double) * ng) == 0) {
+        sp = c;
+        break;
+      }
+    if (sp < 0) {
+      sp = (int32_t)rows.size();
+      rows.push_back(row);
+      cand.push_back(sp);
+    }
+    CallRay& o = cr[j];
+    o.src[0] = R.src.xpos;
+    o.src[1] = R.src.ypos;
+    o.src[2] = R.src.zpos;
+    o.tau[0] = R.optical_depth_HI;
+    o.tau[1] = R.optical_depth_HeI;
+    o.tau[2] = R.optical_depth_HeII;
+    o.spec = sp;
+    o.pad = 0;
+    if (j == 0 || tgt[i] != ct[ntarget - 1].cell) {
+      CallTarget& T = ct[ntarget++];
+      T.first = j;
+      T.cell = tgt[i];
+      T.count = 0;
+      T.g[0] = R.ix_target;
+      T.g[1] = R.iy_target;
+      T.g[2] = R.iz_target;
+    }
+    ct[ntarget - 1].count++;
   }
-  const size_t bytes = sizeof(struct light_ray_IO) * nray;
-  HIP_OK(g.rays.reserve(bytes));
-  HIP_OK(g.per_ray.reserve(6 * nray));
-  HIP_OK(g.target.reserve(nray));
-  HIP_OK(hipMemcpyAsync(g.rays.p, ray_IO, bytes, hipMemcpyHostToDevice, g.stream));
-  const RateTables rt = rate_tables();
-  HIP_OK(launch_ray_rates(gd, rt, g.mesh, g.rays.p, nray, g.per_ray.p, g.target.p, g.stream));
-  HIP_OK(launch_merge_ray_rates(g.per_ray.p, g.target.p, nray, g.mesh, g.stream));
+  const int64_t nspec = (int64_t)rows.size();
+  // The tables, as the per-ray kernels of the fused step read a node's
+  // spectrum: FP32 values scaled by the row maximum, the maximum in FP64,
+  // and the FP64 row for the rays FP32 cannot serve.
+  const size_t off_tab = sizeof(CallRay) * nray + sizeof(CallTarget) * ntarget;
+  const size_t tab_bytes = (sizeof(float) * ng + sizeof(double) * ng + sizeof(double)) * nspec;
+  std::vector<unsigned char> host_tab;
+  unsigned char* tab = stage + off_tab;
+  if (off_tab + tab_bytes > cap) {
+    host_tab.resize(tab_bytes);
+    tab = host_tab.data();
+  }
+  float* s32 = reinterpret_cast<float*>(tab);
+  double* s64 = reinterpret_cast<double*>(tab + sizeof(float) * ng * nspec);
+  double* ssc = s64 + (size_t)ng * nspec;
+  for (int64_t u = 0; u < nspec; u++) {
+    const double* row = rows[u];
+    double mx = 0.0;
+    for (int k = 0; k < ng; k++) mx = std::max(mx, row[k]);
+    const double inv = mx > 0.0 ? 1.0 / mx : 0.0;
+    for (int k = 0; k < ng; k++) s32[u * ng + k] = (float)(row[k] * inv);
+    memcpy(s64 + u * ng, row, sizeof(double) * ng);
+    ssc[u] = mx;
+  }
+  HIP_OK(g.rays.reserve(off_tab + tab_bytes));
+  unsigned char* d = g.rays.p;
+  HIP_OK(hipMemcpyAsync(d, stage, off_tab, hipMemcpyHostToDevice, g.stream));
+  HIP_OK(hipMemcpyAsync(d + off_tab, tab, tab_bytes, hipMemcpyHostToDevice, g.stream));
+  const RateTables rt = rate_tables(g);
+  HIP_OK(hipEventRecord(g.ev_c0, g.stream));
+  HIP_OK(launch_target_rates(
+      gd, rt, g.mesh, reinterpret_cast<const CallTarget*>(d + sizeof(CallRay) * nray), ntarget,
+      reinterpret_cast<const CallRay*>(d), reinterpret_cast<const float*>(d + off_tab),
+      reinterpret_cast<const double*>(d + off_tab + sizeof(float) * ng * nspec),
+      reinterpret_cast<const double*>(d + off_tab + (sizeof(float) + sizeof(double)) * ng * nspec),
+      g.stream));
+  HIP_OK(hipEventRecord(g.ev_c1, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, g.ev_c0, g.ev_c1));
+  g.stats.ms_rate = ms;
+  g.stats.nspec_call = (uint64_t)nspec;
 }
 
 // The entry points whose host buffers grow with the caller's input report
@@ -520,7 +725,9 @@ void zero_out_photoion_rate_cuda(struct cuda_mem_space* cuda_mem, struct cuda_pa
                                  struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   HIP_OK(launch_zero_rates(g.mesh, g.ncell, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
 }
@@ -529,7 +736,10 @@ static void merge_photoion_data_body(struct fluid_mesh* mesh, struct cuda_mem_sp
                                      struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!mesh || !check_ready(cuda_mem, this_run)) return;
+  if (!mesh) return set_err(ARGOT_ERR_ARG);
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   std::vector<float> tab(6 * g.ncell);
   for (int64_t c = 0; c < g.ncell; c++) {
     const prim_chem& p = mesh[c].prev_chem;
@@ -551,7 +761,9 @@ void recv_photoion_data(struct photoion_rate* gamma, struct cuda_mem_space* cuda
                         struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   if (!gamma) return set_err(ARGOT_ERR_ARG);
   if (!cuda_mem->rate_dev) return set_err(ARGOT_ERR_STATE);
   float* rate = reinterpret_cast<float*>(cuda_mem->rate_dev);
@@ -565,7 +777,9 @@ void send_photoion_data(struct photoion_rate* gamma, struct cuda_mem_space* cuda
                         struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   if (!gamma) return set_err(ARGOT_ERR_ARG);
   if (!cuda_mem->rate_dev) return set_err(ARGOT_ERR_STATE);
   float* rate = reinterpret_cast<float*>(cuda_mem->rate_dev);
@@ -589,14 +803,18 @@ void merge_photoion_data(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_me
 // One point-source pass of step_radiation_tree (calc_radiation_transfer
 // with zero_out_photoion_rate_cuda before it): rates into
 // mesh_dev[].prev_chem, counters into g.stats.
-static void point_source_pass(struct radiation_src* src, struct run_param* this_run,
-                              struct cuda_mem_space* cuda_mem) {
+static void point_source_pass(Ctx& g, struct radiation_src* src, struct run_param* this_run) {
   const double t_start = now_ms();
-  if (!check_ready(cuda_mem, this_run)) return;
-  if (this_run->rank_x || this_run->rank_y || this_run->rank_z ||
+  // The fused step covers one rank that owns the whole box: its tree, its
+  // segments (domain faces of the global box) and its walk (local box) are
+  // then over the same box.  A rank of a decomposed mesh would treat the
+  // other ranks' regions as outside the box and miss their columns.
+  if (this_run->rank_x || this_run->rank_y || this_run->rank_z || this_run->nnode_x > 1 ||
+      this_run->nnode_y > 1 || this_run->nnode_z > 1 ||
       this_run->xmin_local != this_run->xmin || this_run->ymin_local != this_run->ymin ||
-      this_run->zmin_local != this_run->zmin)
-    return set_err(ARGOT_ERR_ARG);   // the fused step covers one rank's whole box
+      this_run->zmin_local != this_run->zmin || this_run->xmax_local != this_run->xmax ||
+      this_run->ymax_local != this_run->ymax || this_run->zmax_local != this_run->zmax)
+    return set_err(ARGOT_ERR_ARG);
   const int64_t nsrc = (int64_t)this_run->nsrc;
   if (nsrc < 0 || (nsrc > 0 && !src)) return set_err(ARGOT_ERR_ARG);
   // Rate mode 1 needs a trained network.
@@ -658,7 +876,7 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   a.tree.spec = g.spec.p;
   a.tree.spec_scale = g.spec_scale.p;
   a.tree.src_pos = g.src_pos.p;
-  a.rates = rate_tables();
+  a.rates = rate_tables(g);
   // With no trusted cell every ray would take the exact integral, so the
   // plain kernel runs.
   const int surr = g_rate_mode == 1 && surrogate_any_trusted();
@@ -740,11 +958,15 @@ static void point_source_pass(struct radiation_src* src, struct run_param* this_
   g.stats.nsurrogate = hc.nsurrogate;
   g.stats.nvisit = hc.nvisit;
   g.stats.nemit_iter = hc.nemit_iter;
+  g.stats.noverflow = hc.noverflow;
   g.stats.ms_build = t_built - t_start;
   g.stats.ms_kernel = ms;
   g.stats.ms_total = now_ms() - t_start;
   this_run->nray = hc.nray;
   this_run->nseg = hc.nray;
+  // The reference asserts num_segment < NSEG_PER_RAY for every ray
+  // (calc_ray_segment.c:212); here the step reports it.
+  if (hc.noverflow) set_err(ARGOT_ERR_TOO_MANY_SEGMENTS);
 }
 
 static ChemParams chem_params(const run_param* r) {
@@ -763,7 +985,8 @@ static ChemParams chem_params(const run_param* r) {
 // The chemistry kernel over every cell, timed with HIP events; the maxima
 // of the relative changes as the reference reduces them (max over blocks
 // starting from 0, then MAX(FLT_MIN, .)).
-static void chem_step(const run_param* r, float dtime, float* diff_chem, float* diff_uene) {
+static void chem_step(Ctx& g, const run_param* r, float dtime, float* diff_chem,
+                      float* diff_uene) {
   if (!(r->tunit > 0.0) || !(r->uenetok > 0.0) || !(r->munit > 0.0) || !(r->lunit > 0.0))
     return set_err(ARGOT_ERR_ARG);
   // A cell integrates until its elapsed time reaches dtime * tunit: that
@@ -802,13 +1025,16 @@ static void step_radiation_tree_body(struct fluid_mesh* mesh, struct radiation_s
   (void)this_cuda;
   t_err = ARGOT_OK;
   const double t_start = now_ms();
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   g.stats.iterations = 1;
   g.stats.ms_chem = 0.0;
-  point_source_pass(src, this_run, cuda_mem);
+  point_source_pass(g, src, this_run);
   if (t_err || g_iterations <= 0) return;
   // src/step_radiation_tree.c:666-723: smooth, chemistry, and another pass
   // while the chemistry or the energy still moved by more than 1e-3.
-  uint64_t nray = g.stats.nray, ncross = g.stats.ncross;
+  uint64_t nray = g.stats.nray, ncross = g.stats.ncross, novf = g.stats.noverflow;
   uint64_t nvisit = g.stats.nvisit, nemit = g.stats.nemit_iter;
   double ms_kernel = g.stats.ms_kernel, ms_chem = 0.0;
   int iter = 1;
@@ -823,16 +1049,17 @@ static void step_radiation_tree_body(struct fluid_mesh* mesh, struct radiation_s
     float ms_s = 0.0f;
     HIP_OK(hipEventElapsedTime(&ms_s, g.ev_c0, g.ev_c1));
     float dc = 0.0f, du = 0.0f;
-    chem_step(this_run, dtime, &dc, &du);
+    chem_step(g, this_run, dtime, &dc, &du);
     if (t_err) return;
     ms_chem += ms_s + g.chem_stats.ms_kernel;
     if (!((dc > 1.0e-3 || du > 1.0e-3) && iter < g_iterations)) break;
-    point_source_pass(src, this_run, cuda_mem);
+    point_source_pass(g, src, this_run);
     if (t_err) return;
     nray += g.stats.nray;
     ncross += g.stats.ncross;
     nvisit += g.stats.nvisit;
     nemit += g.stats.nemit_iter;
+    novf += g.stats.noverflow;
     ms_kernel += g.stats.ms_kernel;
     iter++;
   }
@@ -843,6 +1070,7 @@ static void step_radiation_tree_body(struct fluid_mesh* mesh, struct radiation_s
   g.stats.ncross = ncross;
   g.stats.nvisit = nvisit;
   g.stats.nemit_iter = nemit;
+  g.stats.noverflow = novf;
   g.stats.ms_kernel = ms_kernel;
   g.stats.ms_chem = ms_chem;
   g.stats.ms_total = now_ms() - t_start;
@@ -866,8 +1094,10 @@ void step_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this
                         float dtime) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
-  chem_step(this_run, dtime, diff_chem, diff_uene);
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
+  chem_step(g, this_run, dtime, diff_chem, diff_uene);
 }
 
 void step_chemistry(struct cuda_mem_space* cuda_mem, struct cuda_param* this_cuda,
@@ -880,7 +1110,9 @@ double calc_dtime_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* this_c
                       struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return 0.0;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return 0.0;
+  Ctx& g = *cp;
   ChemCounters hc;
   hipError_t e = launch_chem_dtime(g.mesh, g.ncell, chem_params(this_run), g.chem_ctr.p, g.stream);
   if (e == hipSuccess)
@@ -905,13 +1137,16 @@ void update_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* th
                           struct run_param* this_run) {
   (void)this_cuda;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   HIP_OK(launch_chem_update(g.mesh, g.ncell, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
 }
 
+// Statistics of the last call on this thread (its context's).
 void argot_hip_chem_stats(struct argot_chem_stats* out) {
-  if (out) *out = g.chem_stats;
+  if (out) *out = t_last ? t_last->chem_stats : argot_chem_stats{};
 }
 
 void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }
@@ -931,7 +1166,9 @@ static void smooth_photoion_rate_body(struct fluid_mesh* mesh, struct run_param*
   (void)this_cuda;
   (void)this_mpi;
   t_err = ARGOT_OK;
-  if (!check_ready(cuda_mem, this_run)) return;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
   if (this_run->nnode_x > 1 || this_run->nnode_y > 1 || this_run->nnode_z > 1)
     return set_err(ARGOT_ERR_ARG);   // the halo exchange of several ranks is not built
   HIP_OK(g.smooth_in.reserve(6 * (size_t)g.ncell));
@@ -994,7 +1231,7 @@ void argot_hip_autotune_step(struct fluid_mesh* mesh, struct radiation_src* src,
       argot_tune_entry& e = rep->entry[n];
       for (int d = 0; d < 3; d++) e.tile[d] = tl[d];
       e.blocks_per_cu = bpc;
-      e.ms_kernel = g.stats.ms_kernel;
+      e.ms_kernel = find_ctx(cuda_mem)->stats.ms_kernel;
       if (e.ms_kernel < best) {
         best = e.ms_kernel;
         rep->best = n;
@@ -1082,6 +1319,24 @@ int argot_hip_abi_layout(int64_t* out, int n) {
   const int m = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < n && i < m; i++) out[i] = v[i];
   return m;
+}
+
+// The segment builder the fused kernel runs (segment_start), on the host:
+// start point of the segment whose optical depth the ray from src to target
+// cell (ix, iy, iz) keeps, and the number of segments counted up to
+// NSEG_PER_RAY (a return of NSEG_PER_RAY is a ray the step refuses).
+int argot_hip_segment_start(const struct run_param* r, const float* src, int ix, int iy, int iz,
+                            float* start) {
+  const GridDev gd = make_grid(r);
+  const Vec3f s{src[0], src[1], src[2]};
+  const Vec3f t{cell_centre(gd.gxmin, ix, gd.dx), cell_centre(gd.gymin, iy, gd.dy),
+                cell_centre(gd.gzmin, iz, gd.dz)};
+  Vec3f st{};
+  const int n = segment_start(gd, s, t, &st);
+  start[0] = st.x;
+  start[1] = st.y;
+  start[2] = st.z;
+  return n;
 }
 
 // Product tree on the host: build from nsrc sources and list the rays of a

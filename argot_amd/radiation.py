@@ -181,6 +181,19 @@ class PointSourceStep:
         self.send_mesh()
         return self
 
+    def close(self):
+        """Free the library's context for this mesh (argot_hip_release_gpu):
+        its device mesh, buffers and stream."""
+        cmem = getattr(self, "cmem", None)
+        if cmem is not None and cmem.mesh_dev:
+            lib().argot_hip_release_gpu(C.byref(cmem))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def step(self, dtime=0.0, iterations=0):
         """step_radiation_tree: one point-source pass (iterations=0), or the
         reference's transfer/smoothing/chemistry loop of at most

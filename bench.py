@@ -17,8 +17,10 @@ the MLP rate surrogate (training, validated accuracy, rays it may serve at
 directions.
 """
 import argparse
+import hashlib
 import json
 import os
+import platform
 import sys
 import time
 
@@ -37,22 +39,61 @@ def algorithmic_bytes(nray, ncross, ncell):
     return nray * (24 + 12) + 12 * ncross + nray * (12 + 8 + 16) + 24 * ncell
 
 
-def cpu_baseline(w, seconds_target=15.0):
-    """Oracle (plain-C restatement, OpenMP) on contiguous target ranges spread
-    over the box, sized to about seconds_target of host time."""
+# Sources that define the fused kernel's code: a committed profile figure is
+# used only when it was measured on these exact sources.
+KERNEL_SOURCES = ("kernels.hip", "argot_device.h", "argot_internal.h", "kernels.h", "mlp.h",
+                  "surrogate.h", "Makefile")
+
+
+def kernel_source_hash():
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "argot_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def profile_record(name, workload):
+    """(record, None) for a committed rocprofv3 figure of kernel `name` on
+    `workload` (profiles/traffic.json), or (None, reason) when there is none
+    or it was measured on other kernel sources."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None, "no profile"
+    with open(path) as f:
+        rec = json.load(f).get(name)
+    if not rec or rec.get("workload") != workload:
+        return None, f"no profile of {name} on {workload}"
+    if rec.get("source_hash") != kernel_source_hash():
+        return None, "stale: the kernel sources changed after the profile"
+    return rec, None
+
+
+def oracle_problem_for(w, fast=False):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_problem
-    P = oracle_problem(w)
-    n = w.n
+    return oracle_problem(w, fast=fast)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def time_oracle(P, n, seconds_target, threads, nrange=64):
+    """The oracle's step over contiguous target ranges (a quarter x-slab
+    each) spread over the box, until about seconds_target of host time."""
     ncell = n ** 3
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    strip = n * n // 4          # a quarter x-slab per range
-    nrange = 64
+    strip = n * n // 4
     starts = [int(ncell * (k + 0.5) / nrange) // strip * strip for k in range(nrange)]
-    tot_ray = 0
-    tot_t = 0.0
-    done = 0
+    tot_ray, tot_t, done = 0, 0.0, 0
     for s0 in starts:
         t0 = time.perf_counter()
         _, _, st = P.step(s0, min(s0 + strip, ncell), nthreads=threads)
@@ -61,21 +102,85 @@ def cpu_baseline(w, seconds_target=15.0):
         done += 1
         if tot_t > seconds_target:
             break
-    return {
-        "value": tot_ray / tot_t,
-        "unit": "segments/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"{done} ranges of {strip} target cells ({done * strip} of {ncell}), "
-                  f"{tot_ray} segments, {tot_t:.1f} s",
-    }
+    return tot_ray, tot_t, done, strip
+
+
+def cpu_baseline(w, seconds_target=12.0):
+    """The oracle (plain-C restatement, OpenMP on all host cores) on a
+    bounded sample of the workload, built twice: with IEEE flags (the
+    validated checker; `value`) and with the reference's own Makefile flags
+    (-O3 -ffast-math -funroll-loops, src/Makefile:69; `reference_flags`)."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = None
+    for fast in (False, True):
+        P = oracle_problem_for(w, fast=fast)
+        nray, t, done, strip = time_oracle(P, w.n, seconds_target, threads)
+        rec = {"value": nray / t, "unit": "segments/s", "cores": threads, "kind": "port",
+               "sample": f"{done} ranges of {strip} target cells ({done * strip} of {w.n ** 3}), "
+                         f"{nray} segments, {t:.1f} s",
+               "build": "-O3 -ffast-math -funroll-loops" if fast else "-O2 IEEE (no fast-math)"}
+        if not fast:
+            out = rec
+            out["cpu"] = cpu_model()
+        else:
+            out["reference_flags"] = rec
+    return out
+
+
+def fp64_error(w, rates, rec, ntau=20000, size=2048):
+    """Relative error against the FP64 oracle (exact chords, expm1): the
+    step's six per-cell rates on two ranges of target cells (the middle of
+    the box and the cell of the first source), beside the reference
+    arithmetic's own (the oracle's FP32 path); and the FP32 optical depths
+    of ntau captured rays against the exact-chord walk of the same segment."""
+    P = oracle_problem_for(w)
+    ncell = w.n ** 3
+    i, j, k = np.minimum((np.asarray(w.src_pos[0], np.float64) * w.n).astype(np.int64), w.n - 1)
+    starts = [ncell // 2, int((i * w.n + j) * w.n + k) - size // 2]
+    got, r32s, r64s = [], [], []
+    for s0 in starts:
+        s0 = int(min(max(s0, 0), ncell - size))
+        r32, r64, _ = P.step(s0, s0 + size, f64=True)
+        got.append(rates[s0:s0 + size])
+        r32s.append(r32[s0:s0 + size])
+        r64s.append(r64[s0:s0 + size])
+    got, r32, r64 = np.concatenate(got), np.concatenate(r32s), np.concatenate(r64s)
+    fin = np.isfinite(r64).all(1) & np.isfinite(got).all(1) & np.isfinite(r32).all(1)
+    names = ("GammaHI", "HeatHI", "GammaHeI", "GammaHeII", "HeatHeI", "HeatHeII")
+
+    def err(a, b):
+        a = np.asarray(a, np.float64)
+        return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+    eg, er = err(got[fin], r64[fin]), err(r32[fin], r64[fin])
+    out = {"cells": int(fin.sum()),
+           "rates": {n: {"max": float(eg[:, c].max()), "median": float(np.median(eg[:, c])),
+                         "reference_max": float(er[:, c].max()),
+                         "reference_median": float(np.median(er[:, c]))}
+                     for c, n in enumerate(names)}}
+    if rec is not None and len(rec):
+        idx = np.linspace(0, len(rec) - 1, min(ntau, len(rec))).astype(np.int64)
+        n = w.n
+        lo = np.float64(np.float32(0.0))
+        h = np.float64(np.float32(np.float32(1.0) / np.float32(n)))
+        e3 = np.zeros((len(idx), 3))
+        for q, r in enumerate(idx):
+            t = int(rec["target"][r])
+            ijk = np.array([t // (n * n), (t // n) % n, t % n], np.float32).astype(np.float64)
+            end = (lo + (ijk + 0.5) * h).astype(np.float32)
+            t64, _ = P.walk_exact(rec["start"][r], end)
+            d = np.abs(rec["tau"][r].astype(np.float64) - t64)
+            e3[q] = np.where(d == 0.0, 0.0, d / np.maximum(np.abs(t64), 1e-30))
+        out["tau"] = {"rays": int(len(idx)), "max": float(e3.max()),
+                      "median": float(np.median(e3))}
+    return out
 
 
 def log(msg):
     print(f"# {msg}", file=sys.stderr, flush=True)
 
 
-def surrogate_leg(step, plain_seg_s, nray):
+def surrogate_leg(step, plain_seg_s, nray, plain_kernel_ms):
     """Train the rate network on the C4 grid and on the step's own grid
     (bf16-MFMA training on the GPU), report validated accuracy, then run the
     step with rate mode 1 (the network where validated within 1e-3, the
@@ -105,6 +210,17 @@ def surrogate_leg(step, plain_seg_s, nray):
                               "vs_exact": (nray / (ms * 1e-3)) / plain_seg_s,
                               "tolerance": 1e-3}
     log(f"surrogate step: {out['step_rate_mode1']}")
+    # Amdahl ceiling of any rate surrogate: the fused kernel with the
+    # frequency loop taken out (a timing-only build, profiled on the same
+    # workload) bounds how fast a free, perfect surrogate could make it
+    prof, note = profile_record("fused_step_kernel_no_rate_loop", "clumped256")
+    if prof:
+        out["amdahl"] = {"no_rate_loop_kernel_ms": prof["kernel_ms"],
+                         "plain_kernel_ms": plain_kernel_ms,
+                         "max_speedup": plain_kernel_ms / prof["kernel_ms"],
+                         "summary": prof.get("summary")}
+    else:
+        out["amdahl"] = {"note": note}
     return out
 
 
@@ -161,7 +277,8 @@ def walk_leg(step, nsample=8 << 20):
                         "frac": bytes_ / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "tau_equal_to_fused_kernel": same}
     log(f"walk only: {out}")
-    return out
+    sample = rec[np.linspace(0, n - 1, min(n, 20000)).astype(np.int64)].copy()
+    return out, sample
 
 
 def chem_leg(step, dtime=1.0e-3):
@@ -264,7 +381,7 @@ def main():
         print(json.dumps(chem_leg(step)), flush=True)
         return
     if args.walk_only:
-        print(json.dumps(walk_leg(step)), flush=True)
+        print(json.dumps(walk_leg(step)[0]), flush=True)
         return
 
     for i in range(args.warmup):
@@ -305,18 +422,20 @@ def main():
     bytes_launch = algorithmic_bytes(nray, ncross, ncell)
     achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9
     # memory traffic of this kernel on this workload, from the committed
-    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/traffic.json)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath) and not args.kahan:
-        with open(tpath) as f:
-            rec = json.load(f).get("fused_step_kernel", {})
-        if rec.get("workload") == args.workload:
-            traffic = rec.get("traffic_bytes")
+    # rocprofv3 FETCH_SIZE/WRITE_SIZE passes, when they were measured on
+    # the kernel sources of this tree
+    traffic, traffic_note = None, None
+    if args.kahan:
+        traffic_note = "profiled in plain-sum mode only"
+    else:
+        prof, traffic_note = profile_record("fused_step_kernel", args.workload)
+        if prof:
+            traffic = prof.get("traffic_bytes")
     copy_gbs = lib().argot_hip_copy_probe(1 << 30, 10)
     check("argot_hip_copy_probe")
     log(f"copy probe: {copy_gbs:.0f} GB/s (1 GiB copy, read + write)")
     seg_s = nray / (ms_step * 1e-3)
+    rates_plain = step.rates() if not args.no_cpu_baseline else None
     out = {
         "metric": "ray segments/s (point-source step, exact rates)",
         "value": seg_s,
@@ -339,19 +458,79 @@ def main():
                    "tuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_note": traffic_note or "rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                                     "profiles/traffic.json",
                      "kernel": "fused_step_kernel", "kernel_ms": kernel_ms,
                      "bytes_per_launch": int(bytes_launch),
                      "measured_copy_gbs": copy_gbs},
     }
+    rec_sample = None
     if not args.no_extras:
-        out["surrogate"] = surrogate_leg(step, seg_s, nray)
-        out["walk_only"] = walk_leg(step)
+        if not args.kahan:
+            out["kahan"] = step_leg(step, "kahan sums on the same workload", tau_mode=1)
+        out["surrogate"] = surrogate_leg(step, seg_s, nray, kernel_ms)
+        out["walk_only"], rec_sample = walk_leg(step)
         out["chemistry"] = chem_leg(step)
         out["art"] = art_leg()
         log(f"art: {out['art']}")
+        out["lognormal128"] = lognormal128_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)
+        log(f"cpu baseline: {out['cpu_baseline']}")
+        out["fp64_error"] = fp64_error(w, rates_plain, rec_sample)
+        log(f"fp64 error: {out['fp64_error']}")
     print(json.dumps(out), flush=True)
+
+
+def timed_steps(step, nsteps):
+    """nsteps timed calls after one untimed one (graph capture for the
+    current settings): (ms per step, mean kernel ms, last stats)."""
+    step.step()
+    ker = []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        st = step.step()
+        ker.append(st["ms_kernel"])
+    return 1000.0 * (time.perf_counter() - t0) / nsteps, float(np.mean(ker)), st
+
+
+def roofline_of(st, kernel_ms):
+    b = algorithmic_bytes(st["nray"], st["ncross"], st["ntarget"])
+    a = b / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": a / HBM_PEAK_GBS, "bytes_per_launch": int(b)}
+
+
+def step_leg(step, what, tau_mode, nsteps=3):
+    """The headline step with other settings (e.g. Kahan sums), timed like it."""
+    old = step.tau_mode
+    step.tau_mode = tau_mode
+    try:
+        ms, kms, st = timed_steps(step, nsteps)
+    finally:
+        step.tau_mode = old
+    out = {"what": what, "tau_sum": "kahan" if tau_mode else "plain", "steps": nsteps,
+           "segments_per_s": st["nray"] / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": kms,
+           "segments": int(st["nray"]), "crossings": int(st["ncross"]),
+           "roofline": roofline_of(st, kms)}
+    log(f"{what}: {out}")
+    return out
+
+
+def lognormal128_leg(nsteps=3):
+    """The 128^3 lognormal mesh with 1024 power-law sources and compensated
+    optical-depth sums: the step, timed like the headline."""
+    from argot_amd import ic
+    from argot_amd import radiation as R
+    w = ic.lognormal128()
+    step = R.PointSourceStep(w, tau_mode=1)
+    ms, kms, st = timed_steps(step, nsteps)
+    out = {"workload": f"lognormal128: {w.n}^3 mesh, {w.nsrc} sources, kahan sums",
+           "steps": nsteps, "segments_per_s": st["nray"] / (ms * 1e-3), "ms_per_step": ms,
+           "kernel_ms": kms, "segments": int(st["nray"]), "crossings": int(st["ncross"]),
+           "roofline": roofline_of(st, kms)}
+    log(f"lognormal128: {out}")
+    return out
 
 
 if __name__ == "__main__":

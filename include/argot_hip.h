@@ -196,9 +196,21 @@ void setup_cross_section(struct cross_section *csect, struct freq_param *freq);
 /* replaces init_gpu, src/init_gpu.cu:14-145: one device (num_cuda_dev=1),
  * device-resident mesh (AoS, as the reference) and run_param, stream.
  * ARGOT_ERR_ARG for a local mesh with a non-positive size, nx*ny >= 2^24 or
- * more than 2^31-1 cells. */
+ * more than 2^31-1 cells.
+ * Each call creates a context of its own (device mesh, stream, buffers,
+ * step graph) that every later call finds through cuda_mem->mesh_dev: two
+ * meshes, each with its own cuda_mem_space, coexist in one process, and
+ * calls on different cuda_mem_spaces may run concurrently from different
+ * threads (calls on one cuda_mem_space must not overlap).  Calling init_gpu
+ * again on a cuda_mem_space that names a context frees that context first.
+ * The library controls below (argot_hip_set_*) are process-wide. */
 void init_gpu(struct fluid_mesh *mesh, struct cuda_mem_space *cuda_mem,
               struct cuda_param *this_cuda, struct run_param *this_run);
+/* No counterpart in the reference (its process keeps its device memory
+ * until it ends): frees the context init_gpu made for cuda_mem and zeroes
+ * cuda_mem.  Returns ARGOT_OK, or ARGOT_ERR_STATE if cuda_mem names no
+ * context; argot_hip_last_error() is left as it was. */
+int argot_hip_release_gpu(struct cuda_mem_space *cuda_mem);
 /* replaces send_mesh_data, src/send_mesh_data.cu:8-29 */
 void send_mesh_data(struct fluid_mesh *mesh, struct cuda_mem_space *cuda_mem,
                     struct cuda_param *this_cuda, struct run_param *this_run);
@@ -314,7 +326,7 @@ void output_data(struct fluid_mesh *mesh, struct radiation_src *src, struct run_
  * NSOURCE_MAX of src/run_param.h:74). */
 void argot_hip_set_max_sources(uint64_t n);
 
-/* Counters of the last step_chemistry(_gpu) call. */
+/* Counters of the last step_chemistry(_gpu) call on this thread. */
 struct argot_chem_stats {
     uint64_t ncell;          /* cells */
     uint64_t nsub;           /* network sub-steps, summed over cells */
@@ -363,8 +375,14 @@ void merge_photoion_data(struct fluid_mesh *mesh,
 /* replaces step_radiation_tree (GPU variant), src/step_radiation_tree.c:554-733
  * and src/prototype.h:97: tree from src[0..this_run->nsrc), then for every
  * target cell the rays, segments, walk, rates and per-cell sums, fused on
- * the GPU; rates land in mesh_dev[].prev_chem (zeroed first).  Smoothing
- * and the chemistry iteration are outside this library: one pass. */
+ * the GPU; rates land in mesh_dev[].prev_chem (zeroed first).  By default
+ * one point-source pass; argot_hip_set_iterations(n) runs the reference's
+ * transfer / smooth_photoion_rate / step_chemistry loop of at most n passes
+ * and then update_chemistry_gpu.  One rank owning the whole box: a
+ * this_run with nnode_* > 1, a nonzero rank or a local box other than the
+ * global box gives ARGOT_ERR_ARG.  A ray whose source is so far outside the
+ * box that it needs NSEG_PER_RAY or more segments gives
+ * ARGOT_ERR_TOO_MANY_SEGMENTS (the reference asserts). */
 void step_radiation_tree(struct fluid_mesh *mesh, struct radiation_src *src,
                          struct run_param *this_run, struct mpi_param *this_mpi,
                          struct cuda_mem_space *cuda_mem,
@@ -486,7 +504,8 @@ void argot_hip_set_tau_mode(int mode);
  * (default), 1 = MLP surrogate with exact fallback (see below). */
 void argot_hip_set_rate_mode(int mode);
 
-/* Counters of the last step_radiation_tree call. */
+/* Counters of the last step_radiation_tree call on this thread (of the
+ * context that call used). */
 struct argot_step_stats {
     uint64_t ntarget;        /* target cells */
     uint64_t nray;           /* rays (== segments, one rank) */
@@ -505,6 +524,13 @@ struct argot_step_stats {
     double ms_walk;          /* walk kernel of the last calc_optical_depth_cuda
                                 call (segment list already on the device), HIP
                                 events */
+    double ms_rate;          /* rate kernel of the last calc_photoion_rate_cuda
+                                call, HIP events */
+    uint64_t nspec_call;     /* distinct spectra among that call's rays */
+    uint64_t noverflow;      /* rays needing NSEG_PER_RAY or more segments (a
+                                source far outside the box); nonzero makes the
+                                step return ARGOT_ERR_TOO_MANY_SEGMENTS, where
+                                the reference asserts (calc_ray_segment.c:212) */
 };
 void argot_hip_step_stats(struct argot_step_stats *out);
 

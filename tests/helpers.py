@@ -19,13 +19,15 @@ def tables():
     return freq, cs, nu, csa
 
 
-def oracle_problem(w, theta=0.7):
+def oracle_problem(w, theta=0.7, fast=False):
+    """The workload in the oracle's terms; fast=True: the oracle built at the
+    reference's compiler flags (timing only, never a checker)."""
     freq, cs, nu, csa = tables()
     srcs = R.blackbody_sources(w.src_pos, w.src_rate, w.src_temp, freq)
     spec = spectra_of(srcs, w.nsrc)
     return O.Problem(w.dens, w.fHI, w.fHeI, w.fHeII, w.src_pos, spec, lunit=w.lunit,
                      denstonh=w.denstonh, helium_fact=w.helium_fact, theta=theta, nu=nu,
-                     cs=csa)
+                     cs=csa, fast=fast)
 
 
 def small_workload(n=16, nsrc=5, seed=7, lognormal=True, inside_cells=True):
@@ -100,3 +102,45 @@ def library_node_spectra(w, theta=0.7):
                          for u in range(nn)])
     finally:
         L.argot_hip_tree_free(h)
+
+
+RATE_NAMES = ("GammaHI", "HeatHI", "GammaHeI", "GammaHeII", "HeatHeI", "HeatHeII")
+
+
+def fp64_rate_error(x, r64, floor=1e-30):
+    """Per rate: (max, median) relative error of x [N, 6] against the FP64
+    oracle's rates, over cells where both are finite."""
+    x = np.asarray(x, np.float64)
+    fin = np.isfinite(r64).all(1) & np.isfinite(x).all(1)
+    e = rel_err(x[fin], r64[fin], floor)
+    return {n: (float(e[:, k].max()), float(np.median(e[:, k]))) for k, n in enumerate(RATE_NAMES)}
+
+
+def fp64_tau_error(rec, P, shape):
+    """(max, median) relative error of the captured rays' FP32 optical
+    depths (3 species) against the FP64 oracle's exact-chord walk over the
+    same segment (rays with zero optical depth in both count as exact)."""
+    nx, ny, nz = shape
+    errs = np.zeros((len(rec), 3))
+    for i in range(len(rec)):
+        t = int(rec["target"][i])
+        e = np.array([cell_centre(v, n) for v, n in
+                      zip((t // (ny * nz), (t // nz) % ny, t % nz), shape)], np.float32)
+        t64, _ = P.walk_exact(rec["start"][i], e)
+        t32 = rec["tau"][i].astype(np.float64)
+        d = np.abs(t32 - t64)
+        errs[i] = np.where(d == 0.0, 0.0, d / np.maximum(np.abs(t64), 1e-30))
+    return float(errs.max()), float(np.median(errs))
+
+
+def report_fp64(label, gpu, r32, r64, tau=None):
+    """Print the FP64 errors of the GPU rates beside those of the
+    reference's own arithmetic (the oracle's FP32 path); returns both."""
+    g = fp64_rate_error(gpu, r64)
+    r = fp64_rate_error(r32, r64)
+    for n in RATE_NAMES:
+        print(f"{label} {n}: vs FP64 max {g[n][0]:.2e} median {g[n][1]:.2e} "
+              f"(reference arithmetic max {r[n][0]:.2e} median {r[n][1]:.2e})")
+    if tau is not None:
+        print(f"{label} tau: vs FP64 max {tau[0]:.2e} median {tau[1]:.2e}")
+    return g, r

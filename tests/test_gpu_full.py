@@ -14,7 +14,7 @@ import pytest
 
 from argot_amd import ic
 from argot_amd import radiation as R
-from helpers import oracle_problem, rel_err
+from helpers import oracle_problem, rel_err, report_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -48,13 +48,22 @@ def test_full_size_step(require_gpu, name, kahan):
     assert (a[fin] >= 0.0).all()
     assert (~fin.all(axis=1)).sum() <= w.nsrc
     P = oracle_problem(w)
+    g64, r64s, ref32 = [], [], []
     for t0, t1 in sampled_ranges(w):
-        r32, _, _ = P.step(t0, t1, kahan=kahan)
+        r32, r64, _ = P.step(t0, t1, kahan=kahan, f64=True)
         ref, got = r32[t0:t1], a[t0:t1]
         f = np.isfinite(ref)
         assert np.array_equal(f, np.isfinite(got)), f"targets [{t0}, {t1})"
         err = rel_err(got[f], ref[f], RATE_FLOOR)
         assert err.max() < RATE_TOL, f"targets [{t0}, {t1}): max rel err {err.max():.3e}"
+        g64.append(got)
+        ref32.append(ref)
+        r64s.append(r64[t0:t1])
+    # error against the FP64 oracle on the sampled cells, beside the
+    # reference arithmetic's own
+    g, r = report_fp64(name, np.concatenate(g64), np.concatenate(ref32), np.concatenate(r64s))
+    for n in g:
+        assert g[n][0] <= r[n][0] + RATE_TOL, (n, g[n], r[n])
 
 
 def test_full_size_chemistry(require_gpu):

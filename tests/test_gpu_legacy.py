@@ -9,7 +9,7 @@ import pytest
 from argot_amd import abi
 from argot_amd import radiation as R
 from argot_amd._lib import check, lib
-from helpers import cell_centre, oracle_problem, rel_err, small_workload
+from helpers import cell_centre, library_node_spectra, oracle_problem, rel_err, small_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -107,6 +107,86 @@ def test_calc_photoion_rate_cuda(require_gpu):
     assert np.array_equal(fin, np.isfinite(gpu))
     err = rel_err(gpu[fin], r32[fin], 1e-30)
     assert err.max() < 1e-5, err.max()
+
+
+def light_rays_from_records(rec, w, spectra):
+    """light_ray[] of captured fused-step rays (target order, then ray
+    order), each with its node's or source's spectrum row."""
+    n = len(rec)
+    arr = (abi.LightRay * n)()
+    side = w.n
+    t = rec["target"]
+    L = lib()
+    srcs = R.blackbody_sources(w.src_pos, w.src_rate, w.src_temp)
+    src_spec = np.array([np.ctypeslib.as_array(srcs[i].photon_rate) for i in range(w.nsrc)])
+    for i in range(n):
+        r = arr[i]
+        r.src.type = 0
+        r.src.param = 1.0e5
+        r.src.xpos, r.src.ypos, r.src.zpos = (float(v) for v in rec["src"][i])
+        row = spectra[rec["id"][i]] if rec["kind"][i] == 0 else src_spec[rec["id"][i]]
+        np.ctypeslib.as_array(r.src.photon_rate)[:] = row
+        r.ix_target = int(t[i] // (side * side))
+        r.iy_target = int((t[i] // side) % side)
+        r.iz_target = int(t[i] % side)
+        r.optical_depth_HI, r.optical_depth_HeI, r.optical_depth_HeII = (
+            float(v) for v in rec["tau"][i])
+    return arr
+
+
+def rate_call(step, arr, n):
+    L = lib()
+    iop = C.c_void_p()
+    L.allocate_pinned_light_ray_IO(C.byref(iop), n)
+    check("allocate_pinned_light_ray_IO")
+    try:
+        L.zero_out_photoion_rate_cuda(C.byref(step.cmem), C.byref(step.cpar), C.byref(step.run))
+        check("zero_out_photoion_rate_cuda")
+        step.run.nray = n
+        L.calc_photoion_rate_cuda(C.byref(step.cmem), arr, iop, C.byref(step.cpar),
+                                  C.byref(step.run))
+        check("calc_photoion_rate_cuda")
+    finally:
+        L.deallocate_pinned_light_ray_IO(iop)
+    return step.rates()
+
+
+def test_rate_entry_point_equals_fused_step(require_gpu):
+    """Every ray the fused step forms, handed to calc_photoion_rate_cuda
+    with its node's or source's spectrum: the per-target sums in ray order
+    give the fused kernel's per-cell rates bit for bit (cells whose rays all
+    take the FP32 loop); shuffling whole targets, or giving one target's
+    rays in two runs, leaves every cell's sum unchanged."""
+    w = small_workload(n=16, nsrc=9, seed=13)
+    step = R.PointSourceStep(w)
+    rec, st = step.capture(400000)
+    fused = step.rates()
+    spectra = library_node_spectra(w)
+    arr = light_rays_from_records(rec, w, spectra)
+    got = rate_call(step, arr, len(rec))
+    ps = step.stats()
+    assert 0 < ps["nspec_call"] < len(rec) and ps["ms_rate"] > 0.0
+    fin = np.isfinite(fused).all(1)
+    assert fin.sum() >= w.n ** 3 - w.nsrc
+    assert np.array_equal(np.isfinite(got).all(1), fin)
+    assert np.array_equal(got[fin].view(np.uint32), fused[fin].view(np.uint32))
+    # targets in another order: the library regroups nothing within a
+    # target, so each cell's sum is the same
+    t = rec["target"]
+    starts = np.flatnonzero(np.r_[True, t[1:] != t[:-1]])
+    ends = np.r_[starts[1:], len(t)]
+    perm = np.random.default_rng(3).permutation(len(starts))
+    idx = np.concatenate([np.arange(starts[k], ends[k]) for k in perm])
+    arr2 = light_rays_from_records(rec[idx], w, spectra)
+    got2 = rate_call(step, arr2, len(idx))
+    assert np.array_equal(got2[fin].view(np.uint32), fused[fin].view(np.uint32))
+    # the second half of the longest target's rays moved to the end: the
+    # library regroups them after the first half (a stable sort by target)
+    k = int(np.argmax(ends - starts))
+    mid = (starts[k] + ends[k]) // 2
+    idx3 = np.r_[np.arange(0, mid), np.arange(ends[k], len(t)), np.arange(mid, ends[k])]
+    got3 = rate_call(step, light_rays_from_records(rec[idx3], w, spectra), len(idx3))
+    assert np.array_equal(got3[fin].view(np.uint32), fused[fin].view(np.uint32))
 
 
 def test_merge_photoion_data(require_gpu):

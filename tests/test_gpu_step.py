@@ -6,7 +6,8 @@ import pytest
 
 from argot_amd import ic
 from argot_amd import radiation as R
-from helpers import cell_centre, oracle_problem, ragged_workload, rel_err, small_workload
+from helpers import (cell_centre, fp64_tau_error, oracle_problem, ragged_workload, rel_err,
+                     report_fp64, small_workload)
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +66,14 @@ def test_small_mesh_every_ray(require_gpu):
     assert np.array_equal(np.isnan(r32), np.isnan(gpu))
     err = rel_err(gpu[fin], r32[fin], RATE_FLOOR)
     assert err.max() < RATE_TOL, f"max rel err {err.max():.3e}"
+    # error against the FP64 oracle (exact chords, expm1), reported beside
+    # the reference arithmetic's own: the GPU may not be further from FP64
+    # than the reference is, beyond the parity tolerance
+    tau = fp64_tau_error(rec, P, w.dens.shape)
+    g, r = report_fp64("small16", gpu, r32, r64, tau)
+    for n in g:
+        assert g[n][0] <= r[n][0] + RATE_TOL, (n, g[n], r[n])
+    assert tau[0] < 1e-4 and tau[1] < 1e-6
 
 
 def test_small_mesh_kahan(require_gpu):
