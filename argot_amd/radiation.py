@@ -108,6 +108,38 @@ def fill_mesh(w):
     return mesh
 
 
+def light_rays(src_pos, spectra, ijk, tau, temperature=1.0e5):
+    """struct light_ray[n] as setup_light_ray_to leaves them for
+    calc_photoion_rate (src/setup_light_ray_tree.c:321-395): source position
+    and spectrum row, global target cell indices and the optical depths.
+    spectra: [n, 256] rows (or [n] indices into `spectra_table` rows when
+    given as a tuple (table, index))."""
+    n = len(ijk)
+    arr = (abi.LightRay * max(n, 1))()
+    L, S = abi.LightRay, abi.RadiationSrc
+    dt = np.dtype({
+        "names": ["type", "param", "pos", "rate", "ijk", "tau"],
+        "formats": [np.int32, np.float32, (np.float32, 3), (np.float64, abi.NGRID_NU),
+                    (np.int32, 3), (np.float32, 3)],
+        "offsets": [L.src.offset + S.type.offset, L.src.offset + S.param.offset,
+                    L.src.offset + S.xpos.offset, L.src.offset + S.photon_rate.offset,
+                    L.ix_target.offset, L.optical_depth_HI.offset],
+        "itemsize": C.sizeof(L)})
+    v = np.frombuffer((C.c_char * (C.sizeof(L) * n)).from_address(C.addressof(arr)), dtype=dt)
+    v["type"] = 0
+    v["param"] = temperature
+    v["pos"] = np.asarray(src_pos, np.float32)
+    if isinstance(spectra, tuple):
+        table, index = spectra
+        for a in range(0, n, 1 << 16):   # bounded temporaries
+            v["rate"][a:a + (1 << 16)] = table[index[a:a + (1 << 16)]]
+    else:
+        v["rate"] = spectra
+    v["ijk"] = np.asarray(ijk, np.int32)
+    v["tau"] = np.asarray(tau, np.float32)
+    return arr
+
+
 class _StateInfo:
     def __init__(self, n, helium_fact, name):
         self.n, self.helium_fact, self.name = n, helium_fact, name

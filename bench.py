@@ -277,8 +277,75 @@ def walk_leg(step, nsample=8 << 20):
                         "frac": bytes_ / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "tau_equal_to_fused_kernel": same}
     log(f"walk only: {out}")
-    sample = rec[np.linspace(0, n - 1, min(n, 20000)).astype(np.int64)].copy()
-    return out, sample
+    return out, rec
+
+
+def rate_call_leg(step, rec):
+    """calc_photoion_rate_cuda (the reference's per-call rate entry point) on
+    the rays the walk leg captured, grouped by target, each with its tree
+    node's or source's spectrum row: wall time of the call (host regrouping
+    and spectrum deduplication, transfer, kernel) and the rate kernel's own
+    time.  Afterwards the step runs again, so the mesh holds its rates."""
+    import ctypes as C
+    from argot_amd import abi
+    from argot_amd import radiation as R
+    from argot_amd._lib import check, lib
+    L = lib()
+    w, run = step.w, step.run
+    n = len(rec)
+    pos = np.ascontiguousarray(w.src_pos, np.float32)
+    src_rows = np.array([np.ctypeslib.as_array(step.src[i].photon_rate) for i in range(w.nsrc)])
+    box = (C.c_float * 6)(run.xmin, run.ymin, run.zmin, run.xmax, run.ymax, run.zmax)
+    h = L.argot_hip_tree_build(pos.ctypes.data, np.ascontiguousarray(src_rows).ctypes.data,
+                               w.nsrc, src_rows.shape[1], box, 0.7)
+    try:
+        nn = L.argot_hip_tree_nnode(h)
+        node_rows = np.array([np.ctypeslib.as_array(L.argot_hip_tree_spec(h, u),
+                                                    shape=(src_rows.shape[1],))
+                              for u in range(nn)])
+    finally:
+        L.argot_hip_tree_free(h)
+    table = np.concatenate([node_rows, src_rows])
+    index = np.where(rec["kind"] == 0, rec["id"], nn + rec["id"])
+    side = w.n
+    t = rec["target"]
+    ijk = np.stack([t // (side * side), (t // side) % side, t % side], axis=1)
+    arr = R.light_rays(rec["src"], (table, index), ijk, rec["tau"])
+    iop = C.c_void_p()
+    L.allocate_pinned_light_ray_IO(C.byref(iop), n)
+    check("allocate_pinned_light_ray_IO")
+    walls, kms = [], []
+    try:
+        for _ in range(3):
+            L.zero_out_photoion_rate_cuda(C.byref(step.cmem), C.byref(step.cpar), C.byref(run))
+            check("zero_out_photoion_rate_cuda")
+            run.nray = n
+            t0 = time.perf_counter()
+            L.calc_photoion_rate_cuda(C.byref(step.cmem), arr, iop, C.byref(step.cpar),
+                                      C.byref(run))
+            check("calc_photoion_rate_cuda")
+            walls.append(1000.0 * (time.perf_counter() - t0))
+            st = step.stats()
+            kms.append(st["ms_rate"])
+    finally:
+        L.deallocate_pinned_light_ray_IO(iop)
+    del arr
+    step.step()
+    ntarget = len(np.unique(t))
+    kms_ = float(np.mean(kms[1:]))
+    # per ray: optical depths 12 B, target and spectrum index 8 B, target
+    # density and fractions 16 B; per target cell 24 B of rates
+    bytes_ = 36 * n + 24 * ntarget
+    out = {"entry_point": "calc_photoion_rate_cuda", "kernel": "target_rates_kernel",
+           "rays": n, "targets": ntarget, "distinct_spectra": int(st["nspec_call"]),
+           "ms_call": float(np.mean(walls[1:])), "ms_kernel": kms_,
+           "rays_per_s_kernel": n / (kms_ * 1e-3),
+           "rays_per_s_call": n / (np.mean(walls[1:]) * 1e-3),
+           "roofline": {"bound": "valu (FP32 frequency loop, 256 bins per ray)",
+                        "achieved": bytes_ / (kms_ * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": bytes_ / (kms_ * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    log(f"rate call: {out}")
+    return out
 
 
 def chem_leg(step, dtime=1.0e-3):
@@ -352,7 +419,10 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the surrogate, chemistry and ART legs")
+                    help="skip every side leg (the same as --legs none)")
+    ap.add_argument("--legs", default="all",
+                    help="side legs to run, comma-separated: kahan, surrogate, walk, rate_call, "
+                         "chemistry, art, lognormal128 (default: all)")
     ap.add_argument("--art-only", action="store_true",
                     help="run only the ART leg and print its JSON (profiling)")
     ap.add_argument("--chem-only", action="store_true",
@@ -465,14 +535,29 @@ def main():
                      "measured_copy_gbs": copy_gbs},
     }
     rec_sample = None
-    if not args.no_extras:
-        if not args.kahan:
-            out["kahan"] = step_leg(step, "kahan sums on the same workload", tau_mode=1)
+    all_legs = ("kahan", "surrogate", "walk", "rate_call", "chemistry", "art", "lognormal128")
+    legs = set() if args.no_extras or args.legs == "none" else (
+        set(all_legs) if args.legs == "all" else set(args.legs.split(",")))
+    if legs - set(all_legs):
+        raise SystemExit(f"unknown legs {sorted(legs - set(all_legs))}")
+    if "kahan" in legs and not args.kahan:
+        out["kahan"] = step_leg(step, "kahan sums on the same workload", tau_mode=1)
+    if "surrogate" in legs:
         out["surrogate"] = surrogate_leg(step, seg_s, nray, kernel_ms)
-        out["walk_only"], rec_sample = walk_leg(step)
+    if "walk" in legs or "rate_call" in legs:
+        walk, rec = walk_leg(step)
+        if "walk" in legs:
+            out["walk_only"] = walk
+        rec_sample = rec[np.linspace(0, len(rec) - 1, min(len(rec), 20000)).astype(np.int64)]
+        if "rate_call" in legs:
+            out["rate_call"] = rate_call_leg(step, rec)
+        del rec
+    if "chemistry" in legs:
         out["chemistry"] = chem_leg(step)
+    if "art" in legs:
         out["art"] = art_leg()
         log(f"art: {out['art']}")
+    if "lognormal128" in legs:
         out["lognormal128"] = lognormal128_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w)

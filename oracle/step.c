@@ -88,9 +88,11 @@ void orc_step(const orc_grid *g, const orc_fields *f, const orc_tree *t,
                     ncross += (uint64_t)nc;
                     nseg++;
                     if (rates_f64) {
+                        /* the FP64 oracle keeps the same segment: the kept
+                         * one is the ray's path inside the box */
                         double te[3];
                         orc_walk_exact(g, f, s + 3 * q, e + 3 * q, te);
-                        tau64[0] += te[0]; tau64[1] += te[1]; tau64[2] += te[2];
+                        tau64[0] = te[0]; tau64[1] = te[1]; tau64[2] = te[2];
                     }
                 }
                 double tr[3] = {tau32[0], tau32[1], tau32[2]};

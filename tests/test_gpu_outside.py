@@ -72,6 +72,7 @@ def test_sources_outside_every_face(require_gpu):
     print(f"outside sources: max rel err vs FP64 {e64.max():.2e} "
           f"(reference arithmetic {ref64.max():.2e})")
     assert e64.max() <= ref64.max() + RATE_TOL
+    assert e64.max() < 1e-3
 
 
 def test_far_source_reports_too_many_segments(require_gpu):

@@ -153,3 +153,47 @@ def test_step_serves_trusted_rays_within_validated_error(require_gpu):
     # validated error of the trust cells plus the 2-point rule and the
     # interpolation between validation points
     assert rel.max() < 3 * tol, rel.max(0)
+
+
+def test_c4_configuration_and_clumped256(require_gpu):
+    """The surrogate configuration at its stated size: 2^22 samples of the
+    128-bin 13.6 eV - 1 keV integral, validated at 1e-3 on a 32^3 trust
+    grid; then a network on the step's own grid, applied with rate mode 1
+    on the 256^3 / 4096-source mesh.  Rays the network serves must be
+    within 1e-3 of the FP64 rates; where it serves none, rate mode 1 is the
+    exact path bit for bit.  Prints what it serves and the step times."""
+    from argot_amd import ic
+    rep = S.train(c4_config(nsample=1 << 22, batch=16384, nstep=20000, trust_n=32,
+                            trust_tol=1e-3, seed=7))
+    print(f"C4 128-bin: err median {rep['err_median']:.3e} p99 {rep['err_p99']:.3e} "
+          f"trusted {rep['trusted_frac']:.4f} train {rep['ms_train']:.0f} ms", flush=True)
+    assert np.isfinite(rep["err_median"]) and 0.0 <= rep["trusted_frac"] <= 1.0
+    w = ic.clumped256()
+    st = R.PointSourceStep(w, rate_mode=0)
+    exact_stats = st.step()
+    exact = st.rates()
+    rep2 = S.train(step_config(st, nsample=1 << 22, batch=16384, nstep=20000, trust_n=32,
+                               log_tau_lo=-3.0, log_tau_hi=2.5, trust_tol=1e-3, seed=8))
+    st.rate_mode = 1
+    s1 = st.step()
+    got = st.rates()
+    print(f"clumped256 rate mode 1: trusted {rep2['trusted_frac']:.4f}, served "
+          f"{s1['nsurrogate']} of {s1['nray']} rays; kernel {s1['ms_kernel']:.1f} ms "
+          f"(exact {exact_stats['ms_kernel']:.1f} ms)", flush=True)
+    assert s1["nray"] == exact_stats["nray"]
+    if s1["nsurrogate"] == 0:
+        np.testing.assert_array_equal(got.view(np.uint32), exact.view(np.uint32))
+        return
+    rec, _ = st.capture(1 << 21)
+    served = np.nonzero(rec["surrogate"] == 1.0)[0]
+    P = oracle_problem(w)
+    nodes = library_node_spectra(w)
+    spec = lambda i: nodes[rec["id"][i]] if rec["kind"][i] == 0 else P.ray_spec(1, rec["id"][i])
+    n = w.n
+    idx = np.random.default_rng(0).choice(served, min(len(served), 2000), replace=False)
+    ref = np.array([P.ray_rate(rec["src"][i], spec(i), int(rec["target"][i]) // (n * n),
+                               (int(rec["target"][i]) // n) % n, int(rec["target"][i]) % n,
+                               rec["tau"][i].astype(np.float64), mode=1) for i in idx])
+    floor = 1e-6 * np.abs(ref).max(0)
+    rel = np.abs(rec["rate"][idx] - ref) / np.maximum(np.abs(ref), floor)
+    assert rel.max() < 1e-3, rel.max(0)
