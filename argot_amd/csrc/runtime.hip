@@ -203,6 +203,16 @@ void drop_ctx(const void* mesh_dev) {
   if (gone && t_last == gone.get()) t_last = nullptr;
 }
 
+// This thread's last context, if it still exists (another thread may have
+// released it since).
+const Ctx* last_ctx() {
+  if (!t_last) return nullptr;
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  for (auto& c : g_ctx)
+    if (c.get() == t_last) return t_last;
+  return nullptr;
+}
+
 GridDev make_grid(const run_param* r) {
   GridDev d{};   // zeroed padding: launch arguments are compared as bytes
   d.nx = r->nmesh_x_local;
@@ -352,7 +362,8 @@ void argot_hip_set_tau_mode(int mode) { g_tau_mode = mode ? 1 : 0; }
 void argot_hip_set_rate_mode(int mode) { g_rate_mode = mode ? 1 : 0; }
 // Statistics of the last call on this thread (its context's).
 void argot_hip_step_stats(struct argot_step_stats* out) {
-  if (out) *out = t_last ? t_last->stats : argot_step_stats{};
+  const Ctx* c = last_ctx();
+  if (out) *out = c ? c->stats : argot_step_stats{};
 }
 
 // Opening angle used by step_radiation_tree (the reference always sets
@@ -377,7 +388,10 @@ void argot_hip_set_tile(int tx, int ty, int tz) {
 
 void argot_hip_set_graph(int on) { g_use_graph = on ? 1 : 0; }
 
-int64_t argot_hip_graph_captures(void) { return t_last ? t_last->stats_graph_captures : 0; }
+int64_t argot_hip_graph_captures(void) {
+  const Ctx* c = last_ctx();
+  return c ? c->stats_graph_captures : 0;
+}
 
 // Test support: the next step_radiation_tree records every ray it forms
 // (struct RayRecord, 120 B) into host_records[0..cap).
@@ -388,10 +402,9 @@ void argot_hip_set_capture(void* host_records, int64_t cap) {
 
 int64_t argot_hip_record_size(void) { return (int64_t)sizeof(RayRecord); }
 
-void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
-              struct cuda_param* this_cuda, struct run_param* this_run) {
+static void init_gpu_body(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+                          struct cuda_param* this_cuda, struct run_param* this_run) {
   (void)mesh;
-  t_err = ARGOT_OK;
   if (!cuda_mem || !this_cuda || !this_run) return set_err(ARGOT_ERR_ARG);
   // Mesh sizes the kernels' index arithmetic covers: ix*ny + iy in 24 bits
   // (24-bit multiply-adds form the cell index), the cell count in an int
@@ -451,6 +464,18 @@ void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
     g_ctx.push_back(std::move(owned));
   }
   t_last = &g;
+}
+
+void init_gpu(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_mem,
+              struct cuda_param* this_cuda, struct run_param* this_run) {
+  t_err = ARGOT_OK;
+  try {
+    init_gpu_body(mesh, cuda_mem, this_cuda, this_run);
+  } catch (const std::bad_alloc&) {
+    set_err(ARGOT_ERR_NOMEM);
+  } catch (...) {
+    set_err(ARGOT_ERR_ARG);
+  }
 }
 
 // Frees the context init_gpu made for cuda_mem: its device mesh, run_param
@@ -1146,7 +1171,8 @@ void update_chemistry_gpu(struct cuda_mem_space* cuda_mem, struct cuda_param* th
 
 // Statistics of the last call on this thread (its context's).
 void argot_hip_chem_stats(struct argot_chem_stats* out) {
-  if (out) *out = t_last ? t_last->chem_stats : argot_chem_stats{};
+  const Ctx* c = last_ctx();
+  if (out) *out = c ? c->chem_stats : argot_chem_stats{};
 }
 
 void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }

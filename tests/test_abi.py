@@ -99,3 +99,21 @@ def test_flat_dtype_views_match_ctypes():
     a[2].optical_depth_HI = 4.25
     assert v["optical_depth_HI"][2] == np.float32(4.25)
     assert list(v["optical_depth_HeII"]) == [7.0, 8.0, 9.0, 10.0]
+
+
+def test_release_without_context_reports_state_and_keeps_last_error():
+    """argot_hip_release_gpu returns its status and leaves the thread's last
+    error alone (it may run from a destructor between a call and its
+    check); statistics without a context are zero (no GPU call)."""
+    L = lib()
+    run = abi.RunParam()
+    cm = abi.CudaMemSpace()
+    cp = abi.CudaParam()
+    L.zero_out_photoion_rate_cuda(C.byref(cm), C.byref(cp), C.byref(run))
+    assert L.argot_hip_last_error() == 3
+    assert L.argot_hip_release_gpu(C.byref(cm)) == 3
+    assert L.argot_hip_release_gpu(None) == 2
+    assert L.argot_hip_last_error() == 3
+    st = abi.StepStats()
+    L.argot_hip_step_stats(C.byref(st))
+    assert st.nray == 0 and st.ms_kernel == 0.0
