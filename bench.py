@@ -507,7 +507,8 @@ def main():
     seg_s = nray / (ms_step * 1e-3)
     rates_plain = step.rates() if not args.no_cpu_baseline else None
     out = {
-        "metric": "ray segments/s (point-source step, exact rates)",
+        "metric": "ray segments/s + % HBM peak @256^3,4096 src; surrogate vs exact rate @1e-3",
+        "metric_detail": "segments/s of the point-source step with exact rates; % HBM peak in roofline.frac; surrogate vs exact in surrogate",
         "value": seg_s,
         "unit": "segments/s",
         "n_gpus": 1,
