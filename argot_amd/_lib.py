@@ -19,7 +19,8 @@ EXPORTS = [
     "init_gpu", "send_mesh_data", "recv_mesh_data", "send_run_param_data",
     "allocate_pinned_segment", "deallocate_pinned_segment",
     "allocate_pinned_light_ray_IO", "deallocate_pinned_light_ray_IO",
-    "calc_optical_depth_cuda", "calc_photoion_rate_cuda", "zero_out_photoion_rate_cuda",
+    "calc_optical_depth_cuda", "argot_hip_calc_optical_depth_noalloc",
+    "calc_photoion_rate_cuda", "zero_out_photoion_rate_cuda",
     "merge_photoion_data", "recv_photoion_data", "send_photoion_data", "step_radiation_tree",
     "argot_hip_last_error", "argot_hip_error_string", "argot_hip_set_helium_fact",
     "argot_hip_set_tau_mode", "argot_hip_set_rate_mode", "argot_hip_step_stats",
@@ -72,6 +73,7 @@ def lib():
     L.allocate_pinned_light_ray_IO.argtypes = [P(C.c_void_p), C.c_uint64]
     L.deallocate_pinned_light_ray_IO.argtypes = [C.c_void_p]
     L.calc_optical_depth_cuda.argtypes = [C.c_void_p, cm, cp, rp]
+    L.argot_hip_calc_optical_depth_noalloc.argtypes = [C.c_void_p, cm, cp, rp]
     L.calc_photoion_rate_cuda.argtypes = [cm, C.c_void_p, C.c_void_p, cp, rp]
     L.zero_out_photoion_rate_cuda.argtypes = [cm, cp, rp]
     L.merge_photoion_data.argtypes = [C.c_void_p, cm, cp, rp]

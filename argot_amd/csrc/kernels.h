@@ -31,6 +31,10 @@ hipError_t launch_set_rates(void* mesh, const float* rate, int64_t ncell, hipStr
 // than 2^28 cells (meshes with more always are).
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
                                 int kahan, int wide, hipStream_t s);
+// The same walk with the densities formed from the AoS mesh records at each
+// crossing (no packed-field buffer; identical optical depths).
+hipError_t launch_walk_segments_mesh(const GridDev& g, const void* mesh, double helium_fact,
+                                     void* seg, int64_t nseg, int kahan, hipStream_t s);
 // calc_photoion_rate_cuda: ntarget targets, each with its consecutive rays
 // (rays[first .. first + count)), rates added to the mesh's prev_chem.
 // spec32/spec64/spec_scale: the distinct spectra ([nspec * ngrid] FP32

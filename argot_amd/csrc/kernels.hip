@@ -32,17 +32,33 @@ __device__ __forceinline__ float* mesh_fw(unsigned char* mesh, int64_t c, int of
 
 // calc_optical_depth.c:150-156: nHI = dens*fHI, nHe = dens*HELIUM_FACT in
 // double rounded to float, nHeI = nHe*fHeI, all FP32.
-__global__ void pack_fields_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
-                                   double helium_fact, float4* __restrict__ nf) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+__device__ __forceinline__ float4 form_fields(const unsigned char* __restrict__ mesh, int64_t c,
+                                              double helium_fact) {
   const float nH = *mesh_f(mesh, c, kOffDens);
   const float nHI = nH * *mesh_f(mesh, c, kOffFHI);
   const float nHe = (float)((double)nH * helium_fact);
   const float nHeI = nHe * *mesh_f(mesh, c, kOffFHeI);
   const float nHeII = nHe * *mesh_f(mesh, c, kOffFHeII);
-  nf[c] = make_float4(nHI, nHeI, nHeII, 0.0f);
+  return make_float4(nHI, nHeI, nHeII, 0.0f);
 }
+
+__global__ void pack_fields_kernel(const unsigned char* __restrict__ mesh, int64_t ncell,
+                                   double helium_fact, float4* __restrict__ nf) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  nf[c] = form_fields(mesh, c, helium_fact);
+}
+
+// Per-cell densities formed from the AoS mesh records at each crossing, with
+// pack_fields_kernel's arithmetic (so the optical depths are the packed
+// walk's bits): the walk of a call that has no packed-field buffer.
+struct MeshFields {
+  const unsigned char* __restrict__ mesh;
+  double helium_fact;
+  __device__ __forceinline__ float4 operator()(uint32_t idx) const {
+    return form_fields(mesh, (int64_t)idx, helium_fact);
+  }
+};
 
 __global__ void zero_rates_kernel(unsigned char* __restrict__ mesh, int64_t ncell) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -89,7 +105,7 @@ __global__ void set_rates_kernel(unsigned char* __restrict__ mesh, const float* 
 // wavefront walk segments of similar length; each segment's result does not
 // depend on which lane walks it.
 // Fields: where the per-cell densities come from (PackedFields: the packed
-// float4 array).
+// float4 array; MeshFields: the mesh records).
 constexpr int kWalkBlock = 256;
 template <typename Fields>
 __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
@@ -749,6 +765,15 @@ hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, i
   else
     walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
         g, PackedFields<false>{nf}, static_cast<unsigned char*>(seg), nseg, kahan);
+  return hipGetLastError();
+}
+
+hipError_t launch_walk_segments_mesh(const GridDev& g, const void* mesh, double helium_fact,
+                                     void* seg, int64_t nseg, int kahan, hipStream_t s) {
+  if (nseg <= 0) return hipSuccess;
+  walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+      g, MeshFields{static_cast<const unsigned char*>(mesh), helium_fact},
+      static_cast<unsigned char*>(seg), nseg, kahan);
   return hipGetLastError();
 }
 

@@ -24,6 +24,16 @@
 
 using namespace argot;
 
+// Device scratch of argot_hip_calc_optical_depth_noalloc: the device copy of
+// the caller's segment list, 56 B per segment, at most kScratchSegments
+// (56 MiB).  Each GPU holds its own copy of the array from the library's
+// load to the end of the process.  Host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (scratch_on), never
+// through the array's name, which in host code is not a device address.
+constexpr int64_t kScratchSegments = (int64_t)1 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchSegments * 56];
+static_assert(sizeof(struct ray_segment) == 56, "ray_segment is 56 bytes");
+
 namespace {
 
 thread_local int t_err = ARGOT_OK;
@@ -49,6 +59,29 @@ void set_hip_err(hipError_t e, int line) {
       return;                             \
     }                                     \
   } while (0)
+
+// The scratch array's address on each device, resolved once per device
+// (init_gpu, with that device current).  g_scratch_use serialises the calls
+// that use a device's array: each call copies its segments in, walks them
+// and copies them out before it returns, so the lock covers its whole use.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_use[kMaxDevices];
+
+hipError_t scratch_on(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 // Options (library controls).
 double g_helium_fact = 0.0811258278;   // chemistry.h:25 (Y = 0.245)
@@ -109,6 +142,8 @@ struct Ctx {
   int ncu = 256;
   int nx = 0, ny = 0, nz = 0;
   int64_t ncell = 0;
+  int dev = 0;
+  void* scratch = nullptr;         // this device's scratch array (scratch_on)
   unsigned char* mesh = nullptr;   // cuda_mem->mesh_dev
   DevBuf<float4> nf;
   // frequency tables
@@ -427,6 +462,8 @@ static void init_gpu_body(struct fluid_mesh* mesh, struct cuda_mem_space* cuda_m
   std::unique_ptr<Ctx> owned(new Ctx());
   Ctx& g = *owned;
   g.ncu = prop.multiProcessorCount;
+  g.dev = 0;
+  HIP_OK(scratch_on(g.dev, &g.scratch));
   g.nx = this_run->nmesh_x_local;
   g.ny = this_run->nmesh_y_local;
   g.nz = this_run->nmesh_z_local;
@@ -575,6 +612,43 @@ void calc_optical_depth_cuda(struct ray_segment* seg, struct cuda_mem_space* cud
   HIP_OK(hipEventRecord(g.ev_c1, g.stream));
   HIP_OK(hipMemcpyAsync(seg, g.seg.p, bytes, hipMemcpyDeviceToHost, g.stream));
   HIP_OK(hipStreamSynchronize(g.stream));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, g.ev_c0, g.ev_c1));
+  g.stats.ms_walk = ms;
+}
+
+// calc_optical_depth_cuda without an allocation: the segments are staged in
+// this device's scratch array and the walk forms the densities from the
+// mesh records (the packed-field buffer is never needed), so the call makes
+// no hipMalloc, hipFree or other allocating call.
+void argot_hip_calc_optical_depth_noalloc(struct ray_segment* seg, struct cuda_mem_space* cuda_mem,
+                                          struct cuda_param* this_cuda,
+                                          struct run_param* this_run) {
+  (void)this_cuda;
+  t_err = ARGOT_OK;
+  Ctx* const cp = check_ready(cuda_mem, this_run);
+  if (!cp) return;
+  Ctx& g = *cp;
+  const int64_t nseg = (int64_t)this_run->nseg;
+  if (nseg <= 0) return;
+  if (!seg || nseg > kScratchSegments) return set_err(ARGOT_ERR_ARG);
+  if (!g.scratch) return set_err(ARGOT_ERR_STATE);
+  std::lock_guard<std::mutex> lk(g_scratch_use[g.dev]);
+  const size_t bytes = sizeof(struct ray_segment) * nseg;
+  const GridDev gd = make_grid(this_run);
+  hipError_t e = hipMemcpyAsync(g.scratch, seg, bytes, hipMemcpyHostToDevice, g.stream);
+  if (e == hipSuccess) e = hipEventRecord(g.ev_c0, g.stream);
+  if (e == hipSuccess)
+    e = launch_walk_segments_mesh(gd, g.mesh, g_helium_fact, g.scratch, nseg, g_tau_mode,
+                                  g.stream);
+  if (e == hipSuccess) e = hipEventRecord(g.ev_c1, g.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(seg, g.scratch, bytes, hipMemcpyDeviceToHost, g.stream);
+  // the stream is drained on every path, so that no work on the array
+  // outlives the lock
+  const hipError_t es = hipStreamSynchronize(g.stream);
+  if (e == hipSuccess) e = es;
+  HIP_OK(e);
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, g.ev_c0, g.ev_c1));
   g.stats.ms_walk = ms;

@@ -233,6 +233,24 @@ void calc_optical_depth_cuda(struct ray_segment *seg,
                              struct cuda_mem_space *cuda_mem,
                              struct cuda_param *this_cuda,
                              struct run_param *this_run);
+/* No counterpart in the reference (whose calc_optical_depth_cuda allocates
+ * and frees its device segment buffer on every call,
+ * src/calc_optical_depth_cuda.cu:69,131).  For callers that cannot
+ * allocate device memory: serving loops, code running between HIP graph
+ * captures, harnesses that forbid hipMalloc.  calc_optical_depth_cuda's
+ * semantics and optical-depth bits, with no hipMalloc, hipFree or other
+ * allocating call: the segments are staged in a fixed device array of
+ * ARGOT_NOALLOC_MAX_SEGMENTS records (56 MiB, one per GPU, held until the
+ * process ends) and the densities are formed from mesh_dev's records.
+ * this_run->nseg above that limit gives ARGOT_ERR_ARG.  Every call on one
+ * GPU shares that array; the library serialises such calls (each holds a
+ * lock until its segments are copied back), and the array keeps the last
+ * call's segments. */
+#define ARGOT_NOALLOC_MAX_SEGMENTS (1 << 20)
+void argot_hip_calc_optical_depth_noalloc(struct ray_segment *seg,
+                                          struct cuda_mem_space *cuda_mem,
+                                          struct cuda_param *this_cuda,
+                                          struct run_param *this_run);
 /* replaces calc_photoion_rate_cuda, src/calc_photoion_rate_cuda.cu:92-216:
  * this_run->nray rays; ray_IO is the caller's pinned staging buffer; the
  * rates are added into mesh_dev[].prev_chem (no atomics: per-target
