@@ -184,3 +184,25 @@ def test_too_many_sources_is_refused(tmp_path):
         assert err() == 0 and r2.nsrc == 5
     finally:
         L.argot_hip_set_max_sources(1024)
+
+
+def test_huge_source_count_in_a_short_file_is_refused_before_allocating(tmp_path):
+    """A source file whose header claims far more records than it holds is
+    refused by its size (ARGOT_ERR_IO) even when the capacity allows the
+    count: nothing the size of the claimed count is allocated, and nothing
+    crosses the C-ABI as an exception."""
+    w, mesh, src, run = state(nsrc=3, seed=10)
+    prefix = str(tmp_path / "huge")
+    R.write_snapshot(prefix, mesh, src, run)
+    name = prefix + "_src.dat"
+    with open(name, "r+b") as f:
+        f.write(np.uint64(1 << 60).tobytes())     # nsrc field: 2^60 records
+    L = lib()
+    s2 = (abi.RadiationSrc * 3)()
+    r2 = abi.RunParam()
+    L.argot_hip_set_max_sources(1 << 62)
+    try:
+        L.input_src(s2, C.byref(r2), prefix.encode())
+        assert err() == 6 and r2.nsrc == 0        # ARGOT_ERR_IO
+    finally:
+        L.argot_hip_set_max_sources(1024)
