@@ -28,7 +28,8 @@ EXPORTS = [
     "argot_hip_spectral_grid", "argot_hip_surrogate_train", "argot_hip_surrogate_samples",
     "argot_hip_surrogate_get_params", "argot_hip_surrogate_set_params",
     "argot_hip_surrogate_eval", "argot_hip_surrogate_gradient",
-    "argot_hip_surrogate_trust_error",
+    "argot_hip_surrogate_trust_error", "argot_hip_surrogate_get_baseline",
+    "argot_hip_surrogate_set_baseline",
     "init_hd_param", "free_hd_param", "setup_diffuse_photon", "init_diffuse_gpu",
     "send_diffuse_data", "calc_diffuse_photon_radiation", "free_diffuse_gpu",
     "argot_hip_set_diffuse_nside", "argot_hip_set_diffuse_heatcool",
@@ -120,6 +121,9 @@ def lib():
     L.argot_hip_surrogate_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
                                                C.c_void_p]
     L.argot_hip_surrogate_trust_error.argtypes = [C.c_void_p, C.c_int64]
+    L.argot_hip_surrogate_get_baseline.argtypes = [C.c_void_p]
+    L.argot_hip_surrogate_get_baseline.restype = C.c_int
+    L.argot_hip_surrogate_set_baseline.argtypes = [C.c_void_p, C.c_int]
     hd, cd = P(abi.HostDiffuseParam), C.c_void_p
     L.init_hd_param.argtypes = [hd]
     L.free_hd_param.argtypes = [hd]

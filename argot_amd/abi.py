@@ -136,7 +136,9 @@ class SurrogateConfig(C.Structure):
                 ("spec", C.POINTER(C.c_double)), ("nsample", C.c_int64), ("batch", C.c_int),
                 ("nstep", C.c_int), ("lr_max", C.c_float), ("lr_min", C.c_float),
                 ("seed", C.c_uint64), ("log_tau_lo", C.c_float), ("log_tau_hi", C.c_float),
-                ("trust_n", C.c_int), ("trust_tol", C.c_float), ("dtau_max", C.c_float)]
+                ("trust_n", C.c_int), ("trust_tol", C.c_float), ("dtau_max", C.c_float),
+                ("seed_log_tau", C.POINTER(C.c_float)), ("nseed", C.c_int64),
+                ("jitter", C.c_float), ("poly_degree", C.c_int), ("nvalidate", C.c_int64)]
 
 
 class SurrogateReport(C.Structure):
@@ -147,6 +149,7 @@ class SurrogateReport(C.Structure):
 
 
 SURROGATE_NPARAM = 4806
+SURROGATE_NPOLY = 20
 
 
 # One captured ray (test mode of step_radiation_tree), 120 bytes.

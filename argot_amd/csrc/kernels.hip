@@ -561,12 +561,16 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
       }
       bool exact = emit;
       if constexpr (SURR) {
-        // The network runs on the whole wavefront (MFMA) when any lane
-        // can use it; the other lanes take the exact integral below.
+        // The network runs on the whole wavefront (MFMA) and serves the
+        // visit only when every emitting lane can use it; otherwise every
+        // lane takes the exact integral below.  Two 64x64 network
+        // evaluations cost about as much as the exact 256-bin loop, so a
+        // visit that ran both would cost twice as much.
         const double qn = a.surr.node_q[u];
         float xa[3] = {0.0f, 0.0f, 0.0f}, xb[3] = {0.0f, 0.0f, 0.0f};
         const bool use = emit && surr_inputs(a.surr, rt, rr, qn, xa, xb);
-        if (__ballot(use) != 0) {
+        const unsigned long long emitting = __ballot(emit);
+        if (emitting != 0 && __ballot(use) == emitting) {
           float ya[kNout], yb[kNout];
           mlp_forward(a.surr.net, xa[0], xa[1], xa[2], ya, lds);
           mlp_forward(a.surr.net, xb[0], xb[1], xb[2], yb, lds);

@@ -47,13 +47,56 @@ constexpr int kW3T = kW2N + kHidden * kHidden;  // [16 out][64 in]   forward, ro
 constexpr int kW3N = kW3T + 16 * kHidden;       // [64 in][32 out]   backward (dH2), cols >= 6 zero
 constexpr int kNbf16 = kW3N + kHidden * 32;
 
+// Polynomial baseline under the network: the monomials of degree <= 3 of
+// the normalised input, in the order 1, z0, z1, z2, z0z0, z0z1, z0z2, z1z1,
+// z1z2, z2z2, z0z0z0, z0z0z1, z0z0z2, z0z1z1, z0z1z2, z0z2z2, z1z1z1,
+// z1z1z2, z1z2z2, z2z2z2; coefficient m of output o at poly[m * kNout + o].
+constexpr int kNpoly = 20;
+
 // Device view of a trained network.
 struct MlpDev {
   const float* Pq;       // [kNparam] quantized parameters
   const __bf16* Wb;      // [kNbf16] bf16 images
   float xc, xs;          // input normalisation: (log10 tau - xc) * xs
-  float ymean[kNout], ystd[kNout];   // output: log10 F = ymean + ystd * y
+  float ymean[kNout], ystd[kNout];   // output: log10 F = ymean + ystd * y (+ baseline)
+  const float* poly;     // [kNpoly * kNout] baseline coefficients, or unused
+  int npoly;             // 0 (no baseline) or kNpoly
 };
+
+// The baseline at normalised input z: out[o] = sum_m z^m poly[m][o] (FP32,
+// the same operations wherever it is evaluated, so that training targets and
+// inference agree bit for bit).
+__device__ __forceinline__ void poly_eval(const float* __restrict__ c, float z0, float z1,
+                                          float z2, float out[kNout]) {
+  float m[kNpoly];
+  m[0] = 1.0f;
+  m[1] = z0;
+  m[2] = z1;
+  m[3] = z2;
+  m[4] = z0 * z0;
+  m[5] = z0 * z1;
+  m[6] = z0 * z2;
+  m[7] = z1 * z1;
+  m[8] = z1 * z2;
+  m[9] = z2 * z2;
+  m[10] = m[4] * z0;
+  m[11] = m[4] * z1;
+  m[12] = m[4] * z2;
+  m[13] = m[7] * z0;
+  m[14] = m[5] * z2;
+  m[15] = m[9] * z0;
+  m[16] = m[7] * z1;
+  m[17] = m[7] * z2;
+  m[18] = m[9] * z1;
+  m[19] = m[9] * z2;
+#pragma unroll
+  for (int o = 0; o < kNout; o++) {
+    float a = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kNpoly; k++) a = __builtin_fmaf(m[k], c[k * kNout + o], a);
+    out[o] = a;
+  }
+}
 
 __device__ __forceinline__ float sigmoid_f(float z) {
   return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
@@ -134,9 +177,11 @@ __device__ inline void mlp_forward(const MlpDev& net, float x0, float x1, float 
   float* X = lds + 16 * kLdH;
   float* Y = X + 64 * 4;
   const float* P = net.Pq;
-  X[4 * l + 0] = (x0 - net.xc) * net.xs;
-  X[4 * l + 1] = (x1 - net.xc) * net.xs;
-  X[4 * l + 2] = (x2 - net.xc) * net.xs;
+  const float z0 = (x0 - net.xc) * net.xs, z1 = (x1 - net.xc) * net.xs,
+              z2 = (x2 - net.xc) * net.xs;
+  X[4 * l + 0] = z0;
+  X[4 * l + 1] = z1;
+  X[4 * l + 2] = z2;
   __syncthreads();
   const int r = l & 15, c0 = 16 * (l >> 4);
 #pragma unroll 1
@@ -201,6 +246,12 @@ __device__ inline void mlp_forward(const MlpDev& net, float x0, float x1, float 
 #pragma unroll
   for (int o = 0; o < kNout; o++)
     y[o] = net.ymean[o] + net.ystd[o] * (Y[8 * l + o] + P[kOffB3 + o]);
+  if (net.npoly) {
+    float b[kNout];
+    poly_eval(net.poly, z0, z1, z2, b);
+#pragma unroll
+    for (int o = 0; o < kNout; o++) y[o] += b[o];
+  }
   __syncthreads();
 }
 

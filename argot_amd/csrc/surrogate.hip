@@ -64,16 +64,41 @@ __device__ inline float sample_log_tau(uint64_t seed, uint64_t i, int s, double 
   return (float)(lo + (hi - lo) * uniform01(seed, 3 * i + (uint64_t)s));
 }
 
-__global__ void thin_samples_kernel(ThinTables T, int64_t n, uint64_t seed, double lo, double hi,
-                                    float4* __restrict__ X, float* __restrict__ Y) {
+// Samples around seed points (the optical depths a caller's rays occupy):
+// seed j = floor(u * nseed), u uniform under seed ^ 0x5eed5eed, plus jitter
+// times a Box-Muller normal per axis (Box & Muller, Ann. Math. Stat. 29,
+// 1958), clamped to [lo, hi] and rounded to FP32.
+struct SeedDist {
+  const float* seeds;   // [nseed * 3] log10 tau, or null (uniform samples)
+  int64_t nseed;
+  float jitter;
+};
+
+__device__ inline void sample_point(const SeedDist& D, uint64_t seed, uint64_t i, double lo,
+                                    double hi, float x[3]) {
+  if (!D.seeds) {
+    for (int s = 0; s < 3; s++) x[s] = sample_log_tau(seed, i, s, lo, hi);
+    return;
+  }
+  const int64_t j = min((int64_t)(uniform01(seed ^ 0x5eed5eedull, i) * (double)D.nseed),
+                        D.nseed - 1);
+  for (int s = 0; s < 3; s++) {
+    const double u1 = uniform01(seed, 6 * i + 2 * (uint64_t)s);
+    const double u2 = uniform01(seed, 6 * i + 2 * (uint64_t)s + 1);
+    const double z = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * M_PI * u2);
+    const double v = (double)D.seeds[3 * j + s] + (double)D.jitter * z;
+    x[s] = (float)fmin(fmax(v, lo), hi);
+  }
+}
+
+__global__ void thin_samples_kernel(ThinTables T, SeedDist D, int64_t n, uint64_t seed, double lo,
+                                    double hi, float4* __restrict__ X, float* __restrict__ Y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float x[3];
   double N[3], f[kNout];
-  for (int s = 0; s < 3; s++) {
-    x[s] = sample_log_tau(seed, (uint64_t)i, s, lo, hi);
-    N[s] = pow(10.0, (double)x[s]) / T.sig_th[s];
-  }
+  sample_point(D, seed, (uint64_t)i, lo, hi, x);
+  for (int s = 0; s < 3; s++) N[s] = pow(10.0, (double)x[s]) / T.sig_th[s];
   thin_exact(T, N, f);
   X[i] = make_float4(x[0], x[1], x[2], 0.0f);
   for (int o = 0; o < kNout; o++) Y[8 * i + o] = (float)log10(f[o]);
@@ -449,6 +474,53 @@ __global__ __launch_bounds__(64) void mlp_trust_kernel(MlpDev net, ThinTables T,
   if (l == 0) cell_err[c] = e;
 }
 
+// Training targets of a network with a baseline: Y -= baseline(z).
+__global__ void poly_residual_kernel(const float4* __restrict__ X, float* __restrict__ Y,
+                                     int64_t n, float xc, float xs, const float* __restrict__ poly) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 x = X[i];
+  float b[kNout];
+  poly_eval(poly, (x.x - xc) * xs, (x.y - xc) * xs, (x.z - xc) * xs, b);
+  for (int o = 0; o < kNout; o++) Y[8 * i + o] -= b[o];
+}
+
+// Validation with seed points: n points drawn like the training samples
+// (independent counter seed), the network against the exact kernels (FP64);
+// each point's relative error (max over the outputs) goes to pt_err and,
+// as a maximum, to the trust cell it lies in (cell index computed as the
+// step kernel computes it), with a count of points per cell.
+__global__ __launch_bounds__(64) void mlp_validate_points_kernel(
+    MlpDev net, ThinTables T, SeedDist D, int64_t n, uint64_t seed, double lo, double hi, int tn,
+    float flo, float inv_cell, unsigned* __restrict__ cell_bits, unsigned* __restrict__ cell_cnt,
+    float* __restrict__ pt_err, int* __restrict__ pt_cell) {
+  __shared__ float lds[kMlpLds];
+  const int64_t i = 64 * (int64_t)blockIdx.x + threadIdx.x;
+  const int64_t ic = i < n ? i : n - 1;
+  float x[3];
+  double N[3], f[kNout];
+  sample_point(D, seed, (uint64_t)ic, lo, hi, x);
+  for (int s = 0; s < 3; s++) N[s] = pow(10.0, (double)x[s]) / T.sig_th[s];
+  thin_exact(T, N, f);
+  float y[kNout];
+  mlp_forward(net, x[0], x[1], x[2], y, lds);
+  float e = 0.0f;
+  for (int o = 0; o < kNout; o++) {
+    double r = fabs(expm1(((double)y[o] - log10(f[o])) * 2.302585092994046));
+    if (!(r < 1.0e30)) r = 1.0e30;
+    e = fmaxf(e, (float)r);
+  }
+  if (i < n) {
+    pt_err[i] = e;
+    int c[3];
+    for (int s = 0; s < 3; s++) c[s] = min(tn - 1, max(0, (int)((x[s] - flo) * inv_cell)));
+    const int64_t cell = ((int64_t)c[0] * tn + c[1]) * tn + c[2];
+    pt_cell[i] = (int)cell;
+    atomicMax(&cell_bits[cell], __float_as_uint(e));
+    atomicAdd(&cell_cnt[cell], 1u);
+  }
+}
+
 // ------------------------------------------------------------------------
 // Host side.
 
@@ -470,6 +542,8 @@ struct SurrCtx {
   float* cell_err = nullptr;
   int64_t ntrust = 0;        // trust cells
   int64_t ntrusted = 0;      // of which within tolerance
+  float* poly = nullptr;     // [kNpoly * kNout] baseline coefficients (device)
+  int npoly = 0;             // 0: no baseline
 };
 
 SurrCtx s_;
@@ -513,6 +587,8 @@ hipError_t ensure_params() {
   if ((e = dalloc(&s_.v, kNparamPad)) != hipSuccess) return e;
   if ((e = dalloc(&s_.Pq, kNparamPad)) != hipSuccess) return e;
   if ((e = dalloc(&s_.Wb, kNbf16)) != hipSuccess) return e;
+  if ((e = dalloc(&s_.poly, kNpoly * kNout)) != hipSuccess) return e;
+  if ((e = hipMemset(s_.poly, 0, sizeof(float) * kNpoly * kNout)) != hipSuccess) return e;
   return hipMemset(s_.Wb, 0, sizeof(__bf16) * kNbf16);
 }
 
@@ -526,7 +602,77 @@ MlpDev net_dev() {
     d.ymean[o] = s_.ymean[o];
     d.ystd[o] = s_.ystd[o];
   }
+  d.poly = s_.poly;
+  d.npoly = s_.npoly;
   return d;
+}
+
+// Monomials of the baseline at normalised input z, as poly_eval forms them.
+void monomials_host(float z0, float z1, float z2, double m[kNpoly]) {
+  float f[kNpoly];
+  f[0] = 1.0f;
+  f[1] = z0;
+  f[2] = z1;
+  f[3] = z2;
+  f[4] = z0 * z0;
+  f[5] = z0 * z1;
+  f[6] = z0 * z2;
+  f[7] = z1 * z1;
+  f[8] = z1 * z2;
+  f[9] = z2 * z2;
+  f[10] = f[4] * z0;
+  f[11] = f[4] * z1;
+  f[12] = f[4] * z2;
+  f[13] = f[7] * z0;
+  f[14] = f[5] * z2;
+  f[15] = f[9] * z0;
+  f[16] = f[7] * z1;
+  f[17] = f[7] * z2;
+  f[18] = f[9] * z1;
+  f[19] = f[9] * z2;
+  for (int k = 0; k < kNpoly; k++) m[k] = f[k];
+}
+
+// Least-squares cubic baseline of each output over the samples (x, y):
+// normal equations in FP64 with a relative ridge of 1e-12, solved by
+// Gaussian elimination with partial pivoting.  Returns false if singular.
+bool fit_baseline(const std::vector<float4>& x, const std::vector<float>& y, float xc, float xs,
+                  float coef[kNpoly * kNout]) {
+  const int K = kNpoly;
+  std::vector<double> A(K * K, 0.0), B(K * kNout, 0.0);
+  double m[kNpoly];
+  for (size_t i = 0; i < x.size(); i++) {
+    monomials_host((x[i].x - xc) * xs, (x[i].y - xc) * xs, (x[i].z - xc) * xs, m);
+    for (int a = 0; a < K; a++) {
+      for (int b = 0; b < K; b++) A[a * K + b] += m[a] * m[b];
+      for (int o = 0; o < kNout; o++) B[a * kNout + o] += m[a] * y[8 * i + o];
+    }
+  }
+  for (int a = 0; a < K; a++) A[a * K + a] *= 1.0 + 1e-12;
+  for (int col = 0; col < K; col++) {
+    int piv = col;
+    for (int r = col + 1; r < K; r++)
+      if (fabs(A[r * K + col]) > fabs(A[piv * K + col])) piv = r;
+    if (!(fabs(A[piv * K + col]) > 0.0)) return false;
+    if (piv != col) {
+      for (int c = 0; c < K; c++) std::swap(A[col * K + c], A[piv * K + c]);
+      for (int o = 0; o < kNout; o++) std::swap(B[col * kNout + o], B[piv * kNout + o]);
+    }
+    for (int r = 0; r < K; r++) {
+      if (r == col) continue;
+      const double f = A[r * K + col] / A[col * K + col];
+      if (f == 0.0) continue;
+      for (int c = col; c < K; c++) A[r * K + c] -= f * A[col * K + c];
+      for (int o = 0; o < kNout; o++) B[r * kNout + o] -= f * B[col * kNout + o];
+    }
+  }
+  for (int a = 0; a < K; a++)
+    for (int o = 0; o < kNout; o++) {
+      const double v = B[a * kNout + o] / A[a * K + a];
+      if (!std::isfinite(v)) return false;
+      coef[a * kNout + o] = (float)v;
+    }
+  return true;
 }
 
 // Device tables of the exact integral for cfg (freed by the caller).
@@ -628,9 +774,34 @@ void surrogate_node_q(const double* spec64, int nnode, int ngrid, std::vector<do
   }
 }
 
+// The seed distribution of cfg on the device (*dev freed by the caller):
+// ARGOT_OK, or ARGOT_ERR_ARG for an inconsistent one.
+int seed_dist(const argot_surrogate_config* cfg, float** dev, SeedDist* D) {
+  *dev = nullptr;
+  D->seeds = nullptr;
+  D->nseed = 0;
+  D->jitter = 0.0f;
+  if (cfg->nseed == 0) return ARGOT_OK;
+  if (cfg->nseed < 0 || !cfg->seed_log_tau || !(cfg->jitter >= 0.0f) ||
+      !(cfg->jitter < 10.0f))
+    return ARGOT_ERR_ARG;
+  for (int64_t i = 0; i < 3 * cfg->nseed; i++)
+    if (!std::isfinite(cfg->seed_log_tau[i])) return ARGOT_ERR_ARG;
+  if (dalloc(dev, 3 * (size_t)cfg->nseed) != hipSuccess) return ARGOT_ERR_NOMEM;
+  if (hipMemcpy(*dev, cfg->seed_log_tau, sizeof(float) * 3 * cfg->nseed,
+                hipMemcpyHostToDevice) != hipSuccess)
+    return ARGOT_ERR_HIP;
+  D->seeds = *dev;
+  D->nseed = cfg->nseed;
+  D->jitter = cfg->jitter;
+  return ARGOT_OK;
+}
+
 }  // namespace argot
 
 using namespace argot;
+
+static_assert(kNpoly == ARGOT_SURROGATE_NPOLY, "baseline size of the public header");
 
 extern "C" {
 
@@ -639,9 +810,13 @@ static void argot_hip_surrogate_samples_body(const struct argot_surrogate_config
   argot_hip_internal_clear_error();
   S_ARG(cfg && n > 0 && log_tau && log_kernel && cfg->log_tau_lo < cfg->log_tau_hi);
   ThinHost th;
-  const int e = thin_tables(cfg, &th);
+  float* dseed = nullptr;
+  SeedDist D;
+  int e = thin_tables(cfg, &th);
+  if (!e) e = seed_dist(cfg, &dseed, &D);
   if (e) {
     th.release();
+    if (dseed) (void)hipFree(dseed);
     return argot_hip_internal_set_error(e);
   }
   float4* X = nullptr;
@@ -649,10 +824,13 @@ static void argot_hip_surrogate_samples_body(const struct argot_surrogate_config
   hipError_t he = dalloc(&X, n);
   if (he == hipSuccess) he = dalloc(&Y, 8 * (size_t)n);
   if (he == hipSuccess) {
-    thin_samples_kernel<<<(unsigned)((n + 255) / 256), 256>>>(th.t, n, cfg->seed, cfg->log_tau_lo,
-                                                              cfg->log_tau_hi, X, Y);
+    thin_samples_kernel<<<(unsigned)((n + 255) / 256), 256>>>(th.t, D, n, cfg->seed,
+                                                              cfg->log_tau_lo, cfg->log_tau_hi, X,
+                                                              Y);
     he = hipGetLastError();
   }
+  if (he == hipSuccess) he = hipDeviceSynchronize();
+  if (dseed) (void)hipFree(dseed);
   std::vector<float4> hx;
   std::vector<float> hy;
   if (he == hipSuccess) {
@@ -693,29 +871,44 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
   S_ARG(cfg->log_tau_lo < cfg->log_tau_hi && cfg->trust_n >= 1 && cfg->trust_n <= 128);
   S_ARG(cfg->lr_max > 0.0f && cfg->lr_min >= 0.0f && cfg->trust_tol > 0.0f &&
         cfg->dtau_max > 0.0f);
+  S_ARG(cfg->poly_degree == 0 || cfg->poly_degree == 3);
+  S_ARG(cfg->nvalidate >= 0 && cfg->nvalidate <= ((int64_t)1 << 30));
   memset(rep, 0, sizeof(*rep));
   S_OK(ensure_params());
   ThinHost th;
+  float* dseed = nullptr;
+  SeedDist D;
   int e = thin_tables(cfg, &th);
+  if (!e) e = seed_dist(cfg, &dseed, &D);
   if (e) {
     th.release();
+    if (dseed) (void)hipFree(dseed);
     return argot_hip_internal_set_error(e);
   }
+  const bool seeded = D.seeds != nullptr;
   const int64_t n = cfg->nsample;
   const int nwave = cfg->batch / 64;
   const int tn = cfg->trust_n;
   const int64_t ncell = (int64_t)tn * tn * tn;
+  // validation points: 64 per trust cell (uniform), or nvalidate drawn
+  // like the training samples (seeds; default 2^20)
+  const int64_t nval = seeded ? (cfg->nvalidate > 0 ? cfg->nvalidate : ((int64_t)1 << 20))
+                              : 64 * ncell;
   float4* X = nullptr;
   float* Y = nullptr;
   float* partial = nullptr;
   double *loss_part = nullptr, *loss_hist = nullptr;
   float *cell_err = nullptr, *pt_err = nullptr;
+  unsigned* cell_cnt = nullptr;
+  int* pt_cell = nullptr;
   hipError_t he = dalloc(&X, n);
   if (he == hipSuccess) he = dalloc(&Y, 8 * (size_t)n);
   if (he == hipSuccess) he = dalloc(&partial, (size_t)nwave * kNparamPad);
   if (he == hipSuccess) he = dalloc(&loss_part, nwave);
   if (he == hipSuccess) he = dalloc(&loss_hist, cfg->nstep);
-  if (he == hipSuccess) he = dalloc(&pt_err, 64 * (size_t)ncell);
+  if (he == hipSuccess) he = dalloc(&pt_err, (size_t)nval);
+  if (he == hipSuccess && seeded) he = dalloc(&cell_cnt, ncell);
+  if (he == hipSuccess && seeded) he = dalloc(&pt_cell, (size_t)nval);
   if (he == hipSuccess && s_.ntrust != ncell) {
     if (s_.trust) (void)hipFree(s_.trust);
     if (s_.cell_err) (void)hipFree(s_.cell_err);
@@ -734,6 +927,9 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
     if (loss_part) (void)hipFree(loss_part);
     if (loss_hist) (void)hipFree(loss_hist);
     if (pt_err) (void)hipFree(pt_err);
+    if (cell_cnt) (void)hipFree(cell_cnt);
+    if (pt_cell) (void)hipFree(pt_cell);
+    if (dseed) (void)hipFree(dseed);
     th.release();
   };
   if (he != hipSuccess) {
@@ -746,14 +942,41 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
 
   // 1. samples of the exact integral
   double t0 = now_ms();
-  thin_samples_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(th.t, n, cfg->seed,
+  thin_samples_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(th.t, D, n, cfg->seed,
                                                                    cfg->log_tau_lo,
                                                                    cfg->log_tau_hi, X, Y);
   he = hipGetLastError();
   if (he == hipSuccess) he = hipStreamSynchronize(st);
   rep->ms_samples = now_ms() - t0;
+  s_.xc = 0.5f * (cfg->log_tau_lo + cfg->log_tau_hi);
+  s_.xs = 2.0f / (cfg->log_tau_hi - cfg->log_tau_lo);
 
-  // 2. normalisation from the first samples
+  // 2a. cubic baseline (least squares on the first samples); the network
+  // learns what it leaves
+  s_.npoly = 0;
+  if (he == hipSuccess && cfg->poly_degree == 3) {
+    const int64_t m = std::min<int64_t>(n, 65536);
+    std::vector<float4> hx(m);
+    std::vector<float> hy(8 * (size_t)m);
+    he = hipMemcpy(hx.data(), X, sizeof(float4) * m, hipMemcpyDeviceToHost);
+    if (he == hipSuccess) he = hipMemcpy(hy.data(), Y, sizeof(float) * 8 * m, hipMemcpyDeviceToHost);
+    float coef[kNpoly * kNout];
+    if (he == hipSuccess && !fit_baseline(hx, hy, s_.xc, s_.xs, coef)) {
+      cleanup();
+      return argot_hip_internal_set_error(ARGOT_ERR_ARG);
+    }
+    if (he == hipSuccess)
+      he = hipMemcpy(s_.poly, coef, sizeof(coef), hipMemcpyHostToDevice);
+    if (he == hipSuccess) {
+      poly_residual_kernel<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(X, Y, n, s_.xc, s_.xs,
+                                                                        s_.poly);
+      he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he == hipSuccess) s_.npoly = kNpoly;
+  }
+
+  // 2b. normalisation from the first samples
   if (he == hipSuccess) {
     const int64_t m = std::min<int64_t>(n, 65536);
     std::vector<float> hy(8 * (size_t)m);
@@ -769,8 +992,6 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
       s_.ystd[o] = (float)std::max(sqrt(std::max(s2 / m - mu * mu, 0.0)), 1e-6);
     }
   }
-  s_.xc = 0.5f * (cfg->log_tau_lo + cfg->log_tau_hi);
-  s_.xs = 2.0f / (cfg->log_tau_hi - cfg->log_tau_lo);
 
   // 3. initial parameters, Adam state
   if (he == hipSuccess) {
@@ -847,14 +1068,61 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
 
   // 5. validation on the trust grid
   t0 = now_ms();
-  if (he == hipSuccess) {
+  if (he == hipSuccess && seeded) {
+    // points drawn like the samples, binned into the trust cells: a cell is
+    // trusted with at least 8 points, all within the tolerance
+    he = hipMemsetAsync(cell_err, 0, sizeof(float) * ncell, st);
+    if (he == hipSuccess) he = hipMemsetAsync(cell_cnt, 0, sizeof(unsigned) * ncell, st);
+    const float flo = cfg->log_tau_lo;
+    const float inv_cell = tn / (cfg->log_tau_hi - cfg->log_tau_lo);
+    if (he == hipSuccess) {
+      mlp_validate_points_kernel<<<(unsigned)((nval + 63) / 64), 64, 0, st>>>(
+          net, th.t, D, nval, cfg->seed ^ 0x7a11da7eull, cfg->log_tau_lo, cfg->log_tau_hi, tn,
+          flo, inv_cell, reinterpret_cast<unsigned*>(cell_err), cell_cnt, pt_err, pt_cell);
+      he = hipGetLastError();
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    std::vector<float> ce(ncell), pe(nval);
+    std::vector<unsigned> cc(ncell);
+    std::vector<int> pc(nval);
+    if (he == hipSuccess)
+      he = hipMemcpy(ce.data(), cell_err, sizeof(float) * ncell, hipMemcpyDeviceToHost);
+    if (he == hipSuccess)
+      he = hipMemcpy(cc.data(), cell_cnt, sizeof(unsigned) * ncell, hipMemcpyDeviceToHost);
+    if (he == hipSuccess)
+      he = hipMemcpy(pe.data(), pt_err, sizeof(float) * nval, hipMemcpyDeviceToHost);
+    if (he == hipSuccess)
+      he = hipMemcpy(pc.data(), pt_cell, sizeof(int) * nval, hipMemcpyDeviceToHost);
+    if (he == hipSuccess) {
+      std::vector<unsigned char> tr(ncell);
+      int64_t ntr = 0;
+      for (int64_t c = 0; c < ncell; c++) {
+        tr[c] = cc[c] >= 8 && ce[c] <= cfg->trust_tol ? 1 : 0;
+        ntr += tr[c];
+        if (cc[c] == 0) ce[c] = INFINITY;
+      }
+      he = hipMemcpy(s_.trust, tr.data(), ncell, hipMemcpyHostToDevice);
+      if (he == hipSuccess)
+        he = hipMemcpy(cell_err, ce.data(), sizeof(float) * ncell, hipMemcpyHostToDevice);
+      // the fraction of validation points that lie in trusted cells (the
+      // share of such rays the network may serve)
+      int64_t inside = 0;
+      for (int64_t i = 0; i < nval; i++) inside += tr[pc[i]];
+      s_.ntrusted = ntr;
+      std::sort(pe.begin(), pe.end());
+      rep->err_max = pe.back();
+      rep->err_p99 = pe[(size_t)(0.99 * (pe.size() - 1))];
+      rep->err_median = pe[pe.size() / 2];
+      rep->trusted_frac = (double)inside / nval;
+    }
+  } else if (he == hipSuccess) {
     mlp_trust_kernel<<<(unsigned)ncell, 64, 0, st>>>(net, th.t, tn, cfg->log_tau_lo,
                                                       cfg->log_tau_hi, cfg->seed ^ 0x7a11da7eull,
                                                       cell_err, pt_err);
     he = hipGetLastError();
     if (he == hipSuccess) he = hipStreamSynchronize(st);
   }
-  if (he == hipSuccess) {
+  if (he == hipSuccess && !seeded) {
     std::vector<float> ce(ncell), pe(64 * (size_t)ncell);
     he = hipMemcpy(ce.data(), cell_err, sizeof(float) * ncell, hipMemcpyDeviceToHost);
     if (he == hipSuccess)
@@ -931,7 +1199,39 @@ void argot_hip_surrogate_set_params(const float* params, const float* norm) {
   mlp_quantize_kernel<<<(kNparam + 255) / 256, 256, 0, s_.stream>>>(s_.P, s_.Pq, s_.Wb);
   S_OK(hipGetLastError());
   S_OK(hipStreamSynchronize(s_.stream));
+  s_.npoly = 0;         // no baseline until argot_hip_surrogate_set_baseline
   s_.have_net = true;
+  s_.trained = false;   // the trust grid belongs to the trained network
+}
+
+int argot_hip_surrogate_get_baseline(float* coef) {
+  argot_hip_internal_clear_error();
+  if (!coef) {
+    argot_hip_internal_set_error(ARGOT_ERR_ARG);
+    return 0;
+  }
+  if (!s_.have_net) {
+    argot_hip_internal_set_error(ARGOT_ERR_STATE);
+    return 0;
+  }
+  if (s_.npoly == 0) {
+    memset(coef, 0, sizeof(float) * kNpoly * kNout);
+    return 0;
+  }
+  if (hipMemcpy(coef, s_.poly, sizeof(float) * kNpoly * kNout, hipMemcpyDeviceToHost) !=
+      hipSuccess) {
+    argot_hip_internal_set_error(ARGOT_ERR_HIP);
+    return 0;
+  }
+  return s_.npoly;
+}
+
+void argot_hip_surrogate_set_baseline(const float* coef, int npoly) {
+  argot_hip_internal_clear_error();
+  S_ARG(npoly == 0 || (npoly == kNpoly && coef));
+  if (!s_.have_net) return argot_hip_internal_set_error(ARGOT_ERR_STATE);
+  if (npoly) S_OK(hipMemcpy(s_.poly, coef, sizeof(float) * kNpoly * kNout, hipMemcpyHostToDevice));
+  s_.npoly = npoly;
   s_.trained = false;   // the trust grid belongs to the trained network
 }
 

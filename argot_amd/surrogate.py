@@ -48,10 +48,16 @@ class Config:
 
     def __init__(self, nu, cs, spec, nsample=1 << 22, batch=16384, nstep=20000,
                  lr_max=2e-3, lr_min=1e-6, seed=12345, log_tau_lo=-6.0, log_tau_hi=4.0,
-                 trust_n=32, trust_tol=1e-3, dtau_max=0.8):
+                 trust_n=32, trust_tol=1e-3, dtau_max=0.8, seeds=None, jitter=0.1,
+                 poly_degree=0, nvalidate=0):
+        """seeds: log10 threshold optical depths [m,3] the samples are drawn
+        around (Gaussian jitter of std `jitter`); None: uniform over the
+        hull.  poly_degree 3: cubic least-squares baseline under the
+        network."""
         self.nu = np.ascontiguousarray(nu, dtype=np.float64)
         self.cs = np.ascontiguousarray(cs, dtype=np.float32)
         self.spec = np.ascontiguousarray(spec, dtype=np.float64)
+        self.seeds = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.float32)
         self.c = abi.SurrogateConfig(
             ngrid=len(self.nu),
             nu=self.nu.ctypes.data_as(C.POINTER(C.c_double)),
@@ -59,7 +65,11 @@ class Config:
             spec=self.spec.ctypes.data_as(C.POINTER(C.c_double)),
             nsample=nsample, batch=batch, nstep=nstep, lr_max=lr_max, lr_min=lr_min,
             seed=seed, log_tau_lo=log_tau_lo, log_tau_hi=log_tau_hi, trust_n=trust_n,
-            trust_tol=trust_tol, dtau_max=dtau_max)
+            trust_tol=trust_tol, dtau_max=dtau_max,
+            seed_log_tau=(None if self.seeds is None
+                          else self.seeds.ctypes.data_as(C.POINTER(C.c_float))),
+            nseed=0 if self.seeds is None else len(self.seeds), jitter=jitter,
+            poly_degree=poly_degree, nvalidate=nvalidate)
 
 
 def train(cfg):
@@ -120,3 +130,35 @@ def trust_error(trust_n):
     lib().argot_hip_surrogate_trust_error(e.ctypes.data, e.size)
     check("argot_hip_surrogate_trust_error")
     return e.reshape(trust_n, trust_n, trust_n)
+
+
+def get_baseline():
+    """(coefficients [20, 6], monomials in use: 0 or 20) of the polynomial
+    baseline under the network."""
+    c = np.zeros(abi.SURROGATE_NPOLY * NOUT, dtype=np.float32)
+    n = lib().argot_hip_surrogate_get_baseline(c.ctypes.data)
+    check("argot_hip_surrogate_get_baseline")
+    return c.reshape(abi.SURROGATE_NPOLY, NOUT), n
+
+
+def set_baseline(coef):
+    """Set the baseline (coefficients [20, 6]), or remove it (None)."""
+    if coef is None:
+        lib().argot_hip_surrogate_set_baseline(None, 0)
+    else:
+        c = np.ascontiguousarray(coef, dtype=np.float32).reshape(-1)
+        lib().argot_hip_surrogate_set_baseline(c.ctypes.data, abi.SURROGATE_NPOLY)
+    check("argot_hip_surrogate_set_baseline")
+
+
+def seeds_from_rays(tau, run, csect):
+    """log10 threshold optical depths [n,3] of rays with walk optical depths
+    tau [n,3] (the optical_depth_* sums of calc_optical_depth, in code
+    units): tau * lunit * denstonh * sigma_threshold, the network's input
+    scale.  Rays with a zero optical depth in some species are dropped."""
+    cs = np.array([[c.csect_HI, c.csect_HeI, c.csect_HeII] for c in csect[:abi.NGRID_NU]],
+                  dtype=np.float32)
+    sig = np.array([cs[np.nonzero(cs[:, s])[0][0], s] for s in range(3)], dtype=np.float64)
+    t = np.asarray(tau, dtype=np.float64) * (run.lunit * run.denstonh) * sig
+    keep = np.all(t > 0.0, axis=1)
+    return np.log10(t[keep]).astype(np.float32)

@@ -567,7 +567,16 @@ void argot_hip_step_stats(struct argot_step_stats *out);
  * on a trust grid over its input hull: a ray is served by the network only
  * if its spectrum has the trained shape, its path lies in the hull and in
  * trusted cells, and its in-cell optical depth is within dtau_max;
- * otherwise the exact integral runs. */
+ * otherwise the exact integral runs.
+ *
+ * Two sample distributions: uniform over the hull (nseed = 0), or the
+ * optical depths a caller's rays occupy (nseed > 0: each sample is one of
+ * the seed points plus Gaussian jitter, clamped to the hull), with
+ * validation points drawn the same way and binned into the trust grid (a
+ * cell is trusted when at least 8 validation points fall in it and all are
+ * within trust_tol).  poly_degree 3 fits a cubic polynomial of the
+ * normalised input to each output by least squares and trains the network
+ * on the remainder. */
 struct argot_surrogate_config {
     int ngrid;                 /* frequency bins of the integral */
     const double *nu;          /* [ngrid] nu / nu_L */
@@ -582,13 +591,19 @@ struct argot_surrogate_config {
     int trust_n;               /* trust cells per axis */
     float trust_tol;           /* max relative kernel error of a trusted cell */
     float dtau_max;            /* max in-cell optical depth for the 2-point rule */
+    const float *seed_log_tau; /* [nseed*3] log10 threshold optical depths, or NULL */
+    int64_t nseed;             /* seed points (0: uniform samples over the hull) */
+    float jitter;              /* std of the Gaussian jitter around a seed (log10) */
+    int poly_degree;           /* 0: none; 3: cubic baseline under the network */
+    int64_t nvalidate;         /* validation points with seeds (0: 64 per trust cell) */
 };
 struct argot_surrogate_report {
     double loss_first;         /* normalised MSE of the first step's batch */
     double loss_last;          /* mean over the last 1% of steps */
     double ms_samples, ms_train, ms_validate;
-    double err_max, err_p99, err_median;   /* relative kernel error, 64 points per trust cell */
-    double trusted_frac;       /* fraction of trust cells within trust_tol */
+    double err_max, err_p99, err_median;   /* relative kernel error over the validation points */
+    double trusted_frac;       /* uniform: fraction of trust cells within trust_tol;
+                                  seeds: fraction of validation points in trusted cells */
 };
 /* A log frequency grid for the surrogate: ngrid bins over
  * [10^log_nu_min, 10^log_nu_max] nu_L with centres nu[i] (the convention
@@ -612,6 +627,7 @@ void argot_hip_surrogate_samples(const struct argot_surrogate_config *cfg, int64
  * log10 F = ymean + ystd * y). */
 #define ARGOT_SURROGATE_NPARAM 4806
 void argot_hip_surrogate_get_params(float *params, float *norm);
+/* Sets the network (and drops any baseline; see below). */
 void argot_hip_surrogate_set_params(const float *params, const float *norm);
 /* Network outputs (log10 kernels [n*6]) for log10 tau [n*3]. */
 void argot_hip_surrogate_eval(const float *log_tau, float *log_kernel, int64_t n);
@@ -620,8 +636,19 @@ void argot_hip_surrogate_eval(const float *log_tau, float *log_kernel, int64_t n
  * computes it (straight through the bf16 rounding of the weights). */
 void argot_hip_surrogate_gradient(const float *log_tau, const float *log_kernel, int64_t n,
                                   float *grad, double *loss);
-/* Validated relative error per trust cell (trust_n^3 floats, x slowest). */
+/* Validated relative error per trust cell (trust_n^3 floats, x slowest;
+ * with seeds, cells without validation points read as infinity). */
 void argot_hip_surrogate_trust_error(float *err, int64_t n);
+/* The polynomial baseline under the network: ARGOT_SURROGATE_NPOLY
+ * monomials of the normalised input z = (log10 tau - xc) * xs, in the order
+ * 1, z0, z1, z2, z0z0, z0z1, z0z2, z1z1, z1z2, z2z2, z0z0z0, z0z0z1, z0z0z2,
+ * z0z1z1, z0z1z2, z0z2z2, z1z1z1, z1z1z2, z1z2z2, z2z2z2, times coef[m*6 + o]
+ * added to output o.  get returns the number of monomials in use (0 or
+ * ARGOT_SURROGATE_NPOLY); set takes npoly 0 (no baseline) or
+ * ARGOT_SURROGATE_NPOLY. */
+#define ARGOT_SURROGATE_NPOLY 20
+int argot_hip_surrogate_get_baseline(float *coef);
+void argot_hip_surrogate_set_baseline(const float *coef, int npoly);
 
 /* Opening angle for step_radiation_tree (default 0.7 = ARGOT_THETA_CRIT). */
 void argot_hip_set_theta(float theta);

@@ -56,6 +56,53 @@ def sample_log_tau(seed, n, lo, hi):
     return x
 
 
+def sample_seeded(seed, n, seeds, jitter, lo, hi):
+    """The generator's first n inputs with seed points: seed point
+    j = floor(u * nseed) (u uniform under seed ^ 0x5eed5eed), plus jitter
+    times a Box-Muller normal per axis (Box & Muller, Ann. Math. Stat. 29,
+    1958: sqrt(-2 ln(1 - u1)) cos(2 pi u2), counters 6i + 2s and 6i + 2s + 1),
+    clamped to [lo, hi] and rounded to FP32."""
+    seeds = np.asarray(seeds, dtype=np.float32).reshape(-1, 3)
+    i = np.arange(n, dtype=np.uint64)
+    j = np.minimum((uniform01(np.uint64(seed) ^ np.uint64(0x5EED5EED), i) * len(seeds))
+                   .astype(np.int64), len(seeds) - 1)
+    x = np.empty((n, 3), dtype=np.float32)
+    for s in range(3):
+        u1 = uniform01(seed, np.uint64(6) * i + np.uint64(2 * s))
+        u2 = uniform01(seed, np.uint64(6) * i + np.uint64(2 * s + 1))
+        z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * np.pi * u2)
+        v = seeds[j, s].astype(np.float64) + float(np.float32(jitter)) * z
+        x[:, s] = np.clip(v, lo, hi).astype(np.float32)
+    return x
+
+
+NPOLY = 20
+
+
+def monomials(z):
+    """The 20 monomials of degree <= 3 of z [n,3] in the baseline's order:
+    1, z0, z1, z2, then the degree-2 and degree-3 products in
+    lexicographic order of their (sorted) index tuples."""
+    z = np.asarray(z, dtype=np.float64)
+    cols = [np.ones(len(z))]
+    for d in (1, 2, 3):
+        for c in _combos(d):
+            cols.append(np.prod(z[:, list(c)], axis=1))
+    return np.stack(cols, 1)
+
+
+def _combos(d):
+    out = []
+    def rec(start, acc):
+        if len(acc) == d:
+            out.append(tuple(acc))
+            return
+        for k in range(start, 3):
+            rec(k, acc + [k])
+    rec(0, [])
+    return out
+
+
 def sigma_threshold(cs):
     cs = np.asarray(cs, dtype=np.float64).reshape(-1, 3)
     return np.array([cs[np.nonzero(cs[:, s])[0][0], s] for s in range(3)])
@@ -106,15 +153,19 @@ def _silu_grad(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward(Pq, norm, log_tau):
+def forward(Pq, norm, log_tau, poly=None):
     """log10 kernels [n,6] of the network with parameters Pq (already
-    quantized) and normalisation norm = (xc, xs, ymean[6], ystd[6])."""
+    quantized) and normalisation norm = (xc, xs, ymean[6], ystd[6]), plus
+    the polynomial baseline poly [20, 6] of the normalised input if given."""
     W1, b1, W2, b2, W3, b3 = unpack(Pq)
     norm = np.asarray(norm, dtype=np.float64)
     x = (np.asarray(log_tau, dtype=np.float64) - norm[0]) * norm[1]
     h1 = _silu(x @ W1 + b1)
     h2 = _silu(h1 @ W2 + b2)
-    return norm[2:2 + NOUT] + norm[2 + NOUT:] * (h2 @ W3 + b3)
+    y = norm[2:2 + NOUT] + norm[2 + NOUT:] * (h2 @ W3 + b3)
+    if poly is not None:
+        y = y + monomials(x) @ np.asarray(poly, dtype=np.float64).reshape(NPOLY, NOUT)
+    return y
 
 
 def loss_and_grad(Pq, norm, log_tau, log_kernel):

@@ -67,6 +67,45 @@ def test_forward_matches_oracle(require_gpu):
     assert err.max() < 2e-4, err.max()
 
 
+def test_forward_with_baseline_matches_oracle(require_gpu):
+    """The cubic baseline under the network: the GPU forward pass equals
+    the FP64 numpy forward with the same coefficients (normalised error as
+    without a baseline, plus FP32 rounding of the polynomial); set_params
+    drops the baseline again."""
+    P, norm = random_params(4)
+    S.set_params(P, norm)
+    rng = np.random.default_rng(4)
+    poly = rng.normal(scale=0.5, size=(M.NPOLY, M.NOUT)).astype(np.float32)
+    S.set_baseline(poly)
+    coef, npoly = S.get_baseline()
+    assert npoly == M.NPOLY and np.array_equal(coef, poly)
+    x = rng.uniform(-6, 4, (1000, 3)).astype(np.float32)
+    y = S.evaluate(x)
+    yo = M.forward(M.quantize(P), norm, x, poly)
+    err = np.abs(y - yo) / norm[8:]
+    assert err.max() < 2e-4, err.max()
+    S.set_params(P, norm)
+    assert S.get_baseline()[1] == 0
+    np.testing.assert_allclose(S.evaluate(x), M.forward(M.quantize(P), norm, x),
+                               atol=2e-4 * norm[8:].max())
+
+
+def test_seeded_samples_match_oracle(require_gpu):
+    """Samples drawn around seed points: the same inputs as the numpy
+    generator (seed choice, Box-Muller jitter, clamp; to one FP32 ulp, as
+    the libraries' log and cos may differ in the last bit) and the exact
+    kernels at them."""
+    rng = np.random.default_rng(6)
+    seeds = rng.uniform(-2.5, 1.5, (97, 3)).astype(np.float32)
+    cfg = c4_config(seed=41, seeds=seeds, jitter=0.2, log_tau_lo=-3.0, log_tau_hi=2.0)
+    x, y = S.samples(cfg, 4096)
+    xo = M.sample_seeded(41, 4096, seeds, 0.2, -3.0, 2.0)
+    np.testing.assert_allclose(x, xo, rtol=0, atol=4e-7)
+    assert np.mean(x == xo) > 0.99
+    yo = M.kernels(cfg.nu, cfg.cs, cfg.spec, x.astype(np.float64))
+    np.testing.assert_allclose(y, yo, rtol=1e-6, atol=1e-6)
+
+
 def test_gradient_matches_oracle(require_gpu):
     P, norm = random_params(2)
     S.set_params(P, norm)
@@ -158,10 +197,14 @@ def test_step_serves_trusted_rays_within_validated_error(require_gpu):
 def test_c4_configuration_and_clumped256(require_gpu):
     """The surrogate configuration at its stated size: 2^22 samples of the
     128-bin 13.6 eV - 1 keV integral, validated at 1e-3 on a 32^3 trust
-    grid; then a network on the step's own grid, applied with rate mode 1
-    on the 256^3 / 4096-source mesh.  Rays the network serves must be
-    within 1e-3 of the FP64 rates; where it serves none, rate mode 1 is the
-    exact path bit for bit.  Prints what it serves and the step times."""
+    grid over [-6, 4]^3.  Then the network on the step's own grid, trained
+    on 2^22 samples drawn around the optical depths of 2^20 rays the
+    256^3 / 4096-source step emits (captured), with a cubic baseline, and
+    validated on 2^21 points drawn the same way at 5e-4 (the FP32 walk
+    already puts up to 3.6e-4 between the exact path and FP64 on this
+    mesh).  Rate mode 1 must serve rays; the rays it serves are within 1e-3
+    of the FP64 rates, and every cell's rates within 1e-3 of the exact
+    path's.  Prints what it serves and the kernel times."""
     from argot_amd import ic
     rep = S.train(c4_config(nsample=1 << 22, batch=16384, nstep=20000, trust_n=32,
                             trust_tol=1e-3, seed=7))
@@ -171,21 +214,31 @@ def test_c4_configuration_and_clumped256(require_gpu):
     w = ic.clumped256()
     st = R.PointSourceStep(w, rate_mode=0)
     exact_stats = st.step()
-    exact = st.rates()
-    rep2 = S.train(step_config(st, nsample=1 << 22, batch=16384, nstep=20000, trust_n=32,
-                               log_tau_lo=-3.0, log_tau_hi=2.5, trust_tol=1e-3, seed=8))
+    exact = st.rates().astype(np.float64)
+    rec, _ = st.capture(1 << 20)
+    seeds = S.seeds_from_rays(rec["tau"], st.run, st.cs)
+    del rec
+    rep2 = S.train(step_config(st, nsample=1 << 22, batch=16384, nstep=150000, trust_n=64,
+                               log_tau_lo=-3.5, log_tau_hi=3.0, trust_tol=5e-4, seed=8,
+                               seeds=seeds, jitter=0.1, poly_degree=3, nvalidate=1 << 21,
+                               lr_max=3e-3))
     st.rate_mode = 1
     s1 = st.step()
-    got = st.rates()
-    print(f"clumped256 rate mode 1: trusted {rep2['trusted_frac']:.4f}, served "
+    got = st.rates().astype(np.float64)
+    print(f"clumped256 rate mode 1: validated err median {rep2['err_median']:.2e} p99 "
+          f"{rep2['err_p99']:.2e}, trusted {rep2['trusted_frac']:.3f}; served "
           f"{s1['nsurrogate']} of {s1['nray']} rays; kernel {s1['ms_kernel']:.1f} ms "
           f"(exact {exact_stats['ms_kernel']:.1f} ms)", flush=True)
     assert s1["nray"] == exact_stats["nray"]
-    if s1["nsurrogate"] == 0:
-        np.testing.assert_array_equal(got.view(np.uint32), exact.view(np.uint32))
-        return
+    assert s1["nsurrogate"] > 0, rep2
+    fin = np.isfinite(exact) & (exact > 0)
+    assert np.array_equal(np.isfinite(got), np.isfinite(exact))
+    rel_cell = np.abs(got[fin] - exact[fin]) / exact[fin]
+    print(f"per-cell error against the exact path: max {rel_cell.max():.2e}", flush=True)
+    assert rel_cell.max() < 1e-3
     rec, _ = st.capture(1 << 21)
     served = np.nonzero(rec["surrogate"] == 1.0)[0]
+    assert len(served) > 0
     P = oracle_problem(w)
     nodes = library_node_spectra(w)
     spec = lambda i: nodes[rec["id"][i]] if rec["kind"][i] == 0 else P.ray_spec(1, rec["id"][i])
@@ -196,4 +249,6 @@ def test_c4_configuration_and_clumped256(require_gpu):
                                rec["tau"][i].astype(np.float64), mode=1) for i in idx])
     floor = 1e-6 * np.abs(ref).max(0)
     rel = np.abs(rec["rate"][idx] - ref) / np.maximum(np.abs(ref), floor)
+    print(f"served rays against FP64: max {rel.max():.2e} median {np.median(rel):.2e}",
+          flush=True)
     assert rel.max() < 1e-3, rel.max(0)

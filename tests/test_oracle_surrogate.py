@@ -89,3 +89,47 @@ def test_gradient_matches_finite_differences():
         Pm = P.copy(); Pm[i] -= h
         fd = (M.loss_and_grad(Pp, norm, x, y)[0] - M.loss_and_grad(Pm, norm, x, y)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_seeded_samples_pick_seeds_and_jitter_normally():
+    """Zero jitter reproduces the chosen seed points exactly; the seed index
+    is uniform over the seeds; with jitter the offsets are N(0, jitter^2)
+    per axis (mean, standard deviation, and the fraction beyond 2 sigma,
+    which a wrong Box-Muller constant would move); clamping holds."""
+    rng = np.random.default_rng(3)
+    seeds = rng.uniform(-2.0, 1.0, (7, 3)).astype(np.float32)
+    x0 = M.sample_seeded(11, 20000, seeds, 0.0, -6.0, 4.0)
+    match = (x0[:, None, :] == seeds[None, :, :]).all(2)
+    assert match.any(1).all()
+    counts = match.argmax(1)
+    freq = np.bincount(counts, minlength=7) / len(counts)
+    assert np.allclose(freq, 1 / 7, atol=0.015)
+    jit = 0.25
+    x = M.sample_seeded(11, 200000, seeds, jit, -6.0, 4.0).astype(np.float64)
+    # the seed of each sample is the zero-jitter draw with the same counter
+    x0 = M.sample_seeded(11, 200000, seeds, 0.0, -6.0, 4.0).astype(np.float64)
+    d = (x - x0) / jit
+    assert abs(d.mean()) < 0.01
+    assert abs(d.std() - 1.0) < 0.01
+    assert abs(np.mean(np.abs(d) > 2.0) - 0.0455) < 0.003
+    xc = M.sample_seeded(11, 5000, seeds, 3.0, -1.0, 0.5)
+    assert xc.min() >= -1.0 and xc.max() <= 0.5
+
+
+def test_baseline_monomials_and_forward():
+    """The 20 monomials are 1, z and the products of degree 2 and 3 in
+    lexicographic order of their index tuples (checked against
+    itertools), and the forward pass adds monomials(z) @ poly."""
+    import itertools
+    z = np.array([[2.0, 3.0, 5.0], [-1.5, 0.25, 7.0]])
+    want = [np.ones(2)] + [np.prod(z[:, list(c)], axis=1) for d in (1, 2, 3)
+                           for c in itertools.combinations_with_replacement(range(3), d)]
+    assert np.array_equal(M.monomials(z), np.stack(want, 1))
+    assert M.monomials(z)[0, 14] == 2.0 * 3.0 * 5.0          # z0 z1 z2
+    rng = np.random.default_rng(5)
+    P = rng.normal(scale=0.3, size=M.NPARAM)
+    norm = np.concatenate([[-1.0, 0.2], rng.normal(size=6), rng.uniform(1, 2, 6)])
+    poly = rng.normal(size=(M.NPOLY, M.NOUT))
+    x = rng.uniform(-3, 2, (50, 3))
+    diff = M.forward(P, norm, x, poly) - M.forward(P, norm, x)
+    np.testing.assert_allclose(diff, M.monomials((x - norm[0]) * norm[1]) @ poly, rtol=1e-12)
