@@ -494,7 +494,10 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   // live in LDS between rays, so that the walk and the frequency loop have
   // the registers to themselves.
   __shared__ double acc_lds[6][64];
-  __shared__ float4 cell_lds[64];
+  // (the surrogate variant reads the target's fields from the mesh at each
+  // ray instead, so that with the network's LDS it still fits 16 wavefronts
+  // per CU)
+  __shared__ float4 cell_lds[SURR ? 1 : 64];
   float* lds = nullptr;
   if constexpr (SURR) {
     __shared__ float surr_lds[kMlpLds];
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
       if (valid)
         f = make_float4(*mesh_f(a.mesh, cell, kOffDens), *mesh_f(a.mesh, cell, kOffFHI),
                         *mesh_f(a.mesh, cell, kOffFHeI), *mesh_f(a.mesh, cell, kOffFHeII));
-      cell_lds[lane] = f;
+      if constexpr (!SURR) cell_lds[lane] = f;
 #pragma unroll
       for (int k = 0; k < 6; k++) acc_lds[k][lane] = 0.0;
     }
@@ -556,7 +559,12 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
         // where it happens, on the path of sources outside the box
         segment_start(g, src, tgt, &st, &a.ctr->noverflow);
         walk_segment<KAHAN, CAPTURE, NARROW>(g, a.nf, st, tgt, &w);
-        const float4 f = cell_lds[lane];
+        float4 f;
+        if constexpr (SURR)
+          f = make_float4(*mesh_f(a.mesh, cell, kOffDens), *mesh_f(a.mesh, cell, kOffFHI),
+                          *mesh_f(a.mesh, cell, kOffFHeI), *mesh_f(a.mesh, cell, kOffFHeII));
+        else
+          f = cell_lds[lane];
         ray_rate_setup(g, rt, w.tau, src, tgt, f.x, f.y, f.z, f.w, &rr);
       }
       bool exact = emit;

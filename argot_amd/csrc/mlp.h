@@ -162,7 +162,7 @@ __device__ __forceinline__ void lds_store_d(float* m, int ld, int r0, int c0, f3
 }
 
 constexpr int kLdH = 68;                        // padded LDS row (floats)
-constexpr int kMlpLds = 16 * kLdH + 64 * 4 + 64 * 8;   // floats of LDS per wavefront
+constexpr int kMlpLds = 16 * kLdH + 64 * 8;     // floats of LDS per wavefront
 
 // Forward pass for the 64 samples of a wavefront, one per lane; every lane
 // of the wavefront must be active (the MFMAs span the wavefront).  x: this
@@ -174,21 +174,23 @@ __device__ inline void mlp_forward(const MlpDev& net, float x0, float x1, float 
                                    float y[kNout], float* lds) {
   const int l = threadIdx.x & 63;
   float* H = lds;
-  float* X = lds + 16 * kLdH;
-  float* Y = X + 64 * 4;
+  // one 64 x 8 table holds each sample's normalised input (columns 0-2)
+  // and, once layer 1 of its 16-row block has read them, its outputs
+  // (columns 0-5): layer 3 of a block writes only that block's rows
+  float* XY = lds + 16 * kLdH;
   const float* P = net.Pq;
   const float z0 = (x0 - net.xc) * net.xs, z1 = (x1 - net.xc) * net.xs,
               z2 = (x2 - net.xc) * net.xs;
-  X[4 * l + 0] = z0;
-  X[4 * l + 1] = z1;
-  X[4 * l + 2] = z2;
+  XY[8 * l + 0] = z0;
+  XY[8 * l + 1] = z1;
+  XY[8 * l + 2] = z2;
   __syncthreads();
   const int r = l & 15, c0 = 16 * (l >> 4);
 #pragma unroll 1
   for (int mb = 0; mb < 4; mb++) {
     // layer 1: lane computes row r of the block, hidden units c0..c0+15
-    const float a0 = X[4 * (16 * mb + r) + 0], a1 = X[4 * (16 * mb + r) + 1],
-                a2 = X[4 * (16 * mb + r) + 2];
+    const float a0 = XY[8 * (16 * mb + r) + 0], a1 = XY[8 * (16 * mb + r) + 1],
+                a2 = XY[8 * (16 * mb + r) + 2];
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       const int u = c0 + j;
@@ -239,13 +241,13 @@ __device__ inline void mlp_forward(const MlpDev& net, float x0, float x1, float 
     }
     if ((l & 15) < kNout) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) Y[(16 * mb + 4 * (l >> 4) + i) * 8 + (l & 15)] = o[i];
+      for (int i = 0; i < 4; i++) XY[(16 * mb + 4 * (l >> 4) + i) * 8 + (l & 15)] = o[i];
     }
     __syncthreads();
   }
 #pragma unroll
   for (int o = 0; o < kNout; o++)
-    y[o] = net.ymean[o] + net.ystd[o] * (Y[8 * l + o] + P[kOffB3 + o]);
+    y[o] = net.ymean[o] + net.ystd[o] * (XY[8 * l + o] + P[kOffB3 + o]);
   if (net.npoly) {
     float b[kNout];
     poly_eval(net.poly, z0, z1, z2, b);
