@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -671,6 +672,126 @@ struct SpectrumKeyHash {
   }
 };
 
+// Host threads for a pass over n rays: one per 4096 rays, at most 16 and
+// at most the hardware's.
+static int host_threads(int64_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::min<int64_t>(std::min<unsigned>(hw, 16u), std::max<int64_t>(1, n / 4096));
+}
+
+// Runs f(0 .. nt-1), f(0) on the calling thread and the others on threads
+// of their own, and joins them all.  No exception leaves a thread: a share
+// that throws counts as failed.  A share whose thread cannot be started
+// runs on the calling thread.  Returns whether every f(t) returned true.
+extern "C++" template <typename F>
+static bool run_on_threads(int nt, const F& f) {
+  std::vector<unsigned char> ok(nt, 0);
+  auto share = [&](int t) {
+    try {
+      ok[t] = f(t) ? 1 : 0;
+    } catch (...) {
+      ok[t] = 0;
+    }
+  };
+  std::vector<std::thread> th;
+  int started = 1;
+  try {
+    th.reserve(nt);
+    for (; started < nt; started++) th.emplace_back(share, started);
+  } catch (...) {
+  }
+  for (int t = started; t < nt; t++) share(t);
+  share(0);
+  for (auto& x : th) x.join();
+  return std::all_of(ok.begin(), ok.end(), [](unsigned char v) { return v != 0; });
+}
+
+// 64-bit hash of a spectrum row (four independent multiply-xor chains over
+// the row's 64-bit words, then mixed).
+static uint64_t row_hash(const double* row, int n) {
+  uint64_t h[4] = {0x9e3779b97f4a7c15ull, 0xbf58476d1ce4e5b9ull, 0x94d049bb133111ebull,
+                   0xd6e8feb86659fd93ull};
+  for (int k = 0; k < n; k++) {
+    uint64_t v;
+    memcpy(&v, row + k, 8);
+    uint64_t& a = h[k & 3];
+    a = (a ^ v) * 0xff51afd7ed558ccdull;
+    a ^= a >> 29;
+  }
+  uint64_t r = h[0];
+  for (int q = 1; q < 4; q++) r = (r ^ h[q]) * 0xc4ceb9fe1a85ec53ull + (r >> 31);
+  return r;
+}
+
+// Fills the compact ray records cr[j] (ray order[j], or j) and numbers the
+// distinct spectrum rows by first appearance, on up to 16 host threads
+// over contiguous ranges of j: each thread hashes its rows (whole rows),
+// keeps the first ray of each hash and checks every later row of that hash
+// against it byte for byte; the ranges' first rays are then merged in
+// order (checked byte for byte against the row already numbered).  Returns
+// false if two different rows share a hash (the caller then numbers the
+// spectra serially); cr[j].spec is set only on success.
+static bool number_spectra(const light_ray* ray, const int64_t* order, int64_t nray, int ng,
+                           CallRay* cr, std::vector<const double*>* rows) {
+  const int nt = host_threads(nray);
+  std::vector<uint64_t> hs(nray);
+  struct Range {
+    std::vector<uint64_t> h;        // hashes in order of first appearance
+    std::vector<const double*> r;   // their first rows
+    bool clash = false;
+  };
+  std::vector<Range> rg(nt);
+  auto work = [&](int t) -> bool {
+    const int64_t j0 = nray * t / nt, j1 = nray * (t + 1) / nt;
+    std::unordered_map<uint64_t, int32_t> seen;
+    Range& R = rg[t];
+    for (int64_t j = j0; j < j1; j++) {
+      const light_ray& L = ray[order ? order[j] : j];
+      const double* row = L.src.photon_rate;
+      const uint64_t h = row_hash(row, ng);
+      hs[j] = h;
+      auto it = seen.find(h);
+      if (it == seen.end()) {
+        seen.emplace(h, (int32_t)R.h.size());
+        R.h.push_back(h);
+        R.r.push_back(row);
+      } else if (memcmp(R.r[it->second], row, sizeof(double) * ng) != 0) {
+        R.clash = true;
+      }
+      CallRay& o = cr[j];
+      o.src[0] = L.src.xpos;
+      o.src[1] = L.src.ypos;
+      o.src[2] = L.src.zpos;
+      o.tau[0] = L.optical_depth_HI;
+      o.tau[1] = L.optical_depth_HeI;
+      o.tau[2] = L.optical_depth_HeII;
+      o.pad = 0;
+    }
+    return true;
+  };
+  if (!run_on_threads(nt, work)) return false;
+  std::unordered_map<uint64_t, int32_t> index;
+  rows->clear();
+  for (int t = 0; t < nt; t++) {
+    if (rg[t].clash) return false;
+    for (size_t q = 0; q < rg[t].h.size(); q++) {
+      auto it = index.find(rg[t].h[q]);
+      if (it == index.end()) {
+        index.emplace(rg[t].h[q], (int32_t)rows->size());
+        rows->push_back(rg[t].r[q]);
+      } else if (memcmp((*rows)[it->second], rg[t].r[q], sizeof(double) * ng) != 0) {
+        return false;
+      }
+    }
+  }
+  auto label = [&](int t) -> bool {
+    const int64_t j0 = nray * t / nt, j1 = nray * (t + 1) / nt;
+    for (int64_t j = j0; j < j1; j++) cr[j].spec = index.find(hs[j])->second;
+    return true;
+  };
+  return run_on_threads(nt, label);
+}
+
 static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct light_ray* ray,
                                          struct light_ray_IO* ray_IO, struct cuda_param* this_cuda,
                                          struct run_param* this_run) {
@@ -689,12 +810,20 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   // (calc_photoion_rate_cuda.cu:108-132 repacks them; the per-target sums
   // then need no atomics).
   std::vector<int64_t> tgt(nray);
-  for (int64_t i = 0; i < nray; i++) {
-    const int64_t lx = ray[i].ix_target - gd.ox, ly = ray[i].iy_target - gd.oy,
-                  lz = ray[i].iz_target - gd.oz;
-    if (lx < 0 || lx >= g.nx || ly < 0 || ly >= g.ny || lz < 0 || lz >= g.nz)
-      return set_err(ARGOT_ERR_ARG);
-    tgt[i] = lz + (int64_t)g.nz * (ly + (int64_t)g.ny * lx);
+  {
+    // target cells on the same host threads as the spectra below
+    const int nt = host_threads(nray);
+    auto cells = [&](int t) -> bool {
+      const int64_t i0 = nray * t / nt, i1 = nray * (t + 1) / nt;
+      for (int64_t i = i0; i < i1; i++) {
+        const int64_t lx = ray[i].ix_target - gd.ox, ly = ray[i].iy_target - gd.oy,
+                      lz = ray[i].iz_target - gd.oz;
+        if (lx < 0 || lx >= g.nx || ly < 0 || ly >= g.ny || lz < 0 || lz >= g.nz) return false;
+        tgt[i] = lz + (int64_t)g.nz * (ly + (int64_t)g.ny * lx);
+      }
+      return true;
+    };
+    if (!run_on_threads(nt, cells)) return set_err(ARGOT_ERR_ARG);
   }
   bool contiguous = true;
   {
@@ -720,38 +849,37 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   CallRay* cr = reinterpret_cast<CallRay*>(stage);
   CallTarget* ct = reinterpret_cast<CallTarget*>(stage + sizeof(CallRay) * nray);
   int64_t ntarget = 0;
-  std::unordered_map<SpectrumKey, std::vector<int32_t>, SpectrumKeyHash> index;
   std::vector<const double*> rows;   // first ray's row of each distinct spectrum
+  if (!number_spectra(ray, contiguous ? nullptr : order.data(), nray, ng, cr, &rows)) {
+    // two different rows with one hash: number them serially, candidates
+    // found by three bins and confirmed on the whole row
+    std::unordered_map<SpectrumKey, std::vector<int32_t>, SpectrumKeyHash> index;
+    rows.clear();
+    for (int64_t j = 0; j < nray; j++) {
+      const double* row = ray[contiguous ? j : order[j]].src.photon_rate;
+      SpectrumKey key;
+      memcpy(&key.a, &row[0], 8);
+      memcpy(&key.b, &row[ng / 2], 8);
+      memcpy(&key.c, &row[ng - 1], 8);
+      std::vector<int32_t>& cand = index[key];
+      int32_t sp = -1;
+      for (int32_t c : cand)
+        if (memcmp(rows[c], row, sizeof(double) * ng) == 0) {
+          sp = c;
+          break;
+        }
+      if (sp < 0) {
+        sp = (int32_t)rows.size();
+        rows.push_back(row);
+        cand.push_back(sp);
+      }
+      cr[j].spec = sp;
+    }
+  }
   for (int64_t j = 0; j < nray; j++) {
     const int64_t i = contiguous ? j : order[j];
-    const light_ray& R = ray[i];
-    const double* row = R.src.photon_rate;
-    SpectrumKey key;
-    memcpy(&key.a, &row[0], 8);
-    memcpy(&key.b, &row[ng / 2], 8);
-    memcpy(&key.c, &row[ng - 1], 8);
-    std::vector<int32_t>& cand = index[key];
-    int32_t sp = -1;
-    for (int32_t c : cand)
-      if (memcmp(rows[c], row, sizeof(double) * ng) == 0) {
-        sp = c;
-        break;
-      }
-    if (sp < 0) {
-      sp = (int32_t)rows.size();
-      rows.push_back(row);
-      cand.push_back(sp);
-    }
-    CallRay& o = cr[j];
-    o.src[0] = R.src.xpos;
-    o.src[1] = R.src.ypos;
-    o.src[2] = R.src.zpos;
-    o.tau[0] = R.optical_depth_HI;
-    o.tau[1] = R.optical_depth_HeI;
-    o.tau[2] = R.optical_depth_HeII;
-    o.spec = sp;
-    o.pad = 0;
     if (j == 0 || tgt[i] != ct[ntarget - 1].cell) {
+      const light_ray& R = ray[i];
       CallTarget& T = ct[ntarget++];
       T.first = j;
       T.cell = tgt[i];
