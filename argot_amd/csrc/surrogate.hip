@@ -948,8 +948,26 @@ static void argot_hip_surrogate_train_body(const struct argot_surrogate_config* 
   he = hipGetLastError();
   if (he == hipSuccess) he = hipStreamSynchronize(st);
   rep->ms_samples = now_ms() - t0;
+  // input normalisation: the hull mapped to [-1, 1]; with seed points the
+  // mean and spread of their coordinates instead (one pair for the three
+  // axes), so that the network's inputs span the range the rays occupy
   s_.xc = 0.5f * (cfg->log_tau_lo + cfg->log_tau_hi);
   s_.xs = 2.0f / (cfg->log_tau_hi - cfg->log_tau_lo);
+  if (seeded) {
+    double m1 = 0.0, m2 = 0.0;
+    const int64_t m = 3 * cfg->nseed;
+    for (int64_t i = 0; i < m; i++) m1 += cfg->seed_log_tau[i];
+    m1 /= (double)m;
+    for (int64_t i = 0; i < m; i++) {
+      const double d = cfg->seed_log_tau[i] - m1;
+      m2 += d * d;
+    }
+    const double sd = sqrt(m2 / (double)m) + (double)cfg->jitter;
+    if (sd > 0.0 && std::isfinite(sd)) {
+      s_.xc = (float)m1;
+      s_.xs = (float)(1.0 / (2.0 * sd));
+    }
+  }
 
   // 2a. cubic baseline (least squares on the first samples); the network
   // learns what it leaves
