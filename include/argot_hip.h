@@ -519,7 +519,11 @@ void argot_hip_set_helium_fact(double helium_fact);
 void argot_hip_set_tau_mode(int mode);
 
 /* Rate integral of step_radiation_tree: 0 = exact frequency integral
- * (default), 1 = MLP surrogate with exact fallback (see below). */
+ * (default), 1 = MLP surrogate with exact fallback (see below): a
+ * wavefront's visit of a tree node is served by the network only when every
+ * emitting lane's ray qualifies, otherwise every lane takes the exact
+ * integral, so per-cell rates depend on the target tile shape within the
+ * validated tolerance. */
 void argot_hip_set_rate_mode(int mode);
 
 /* Counters of the last step_radiation_tree call on this thread (of the
