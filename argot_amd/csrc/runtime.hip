@@ -729,8 +729,8 @@ static uint64_t row_hash(const double* row, int n) {
 // keeps the first ray of each hash and checks every later row of that hash
 // against it byte for byte; the ranges' first rays are then merged in
 // order (checked byte for byte against the row already numbered).  Returns
-// false if two different rows share a hash (the caller then numbers the
-// spectra serially); cr[j].spec is set only on success.
+// false if two different rows share a hash or a thread failed (the caller
+// then fills the records and numbers the spectra serially).
 static bool number_spectra(const light_ray* ray, const int64_t* order, int64_t nray, int ng,
                            CallRay* cr, std::vector<const double*>* rows) {
   const int nt = host_threads(nray);
@@ -851,12 +851,22 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   int64_t ntarget = 0;
   std::vector<const double*> rows;   // first ray's row of each distinct spectrum
   if (!number_spectra(ray, contiguous ? nullptr : order.data(), nray, ng, cr, &rows)) {
-    // two different rows with one hash: number them serially, candidates
-    // found by three bins and confirmed on the whole row
+    // two different rows with one hash, or a host thread that failed:
+    // the records and the numbering again, serially, candidates found by
+    // three bins and confirmed on the whole row
     std::unordered_map<SpectrumKey, std::vector<int32_t>, SpectrumKeyHash> index;
     rows.clear();
     for (int64_t j = 0; j < nray; j++) {
-      const double* row = ray[contiguous ? j : order[j]].src.photon_rate;
+      const light_ray& L = ray[contiguous ? j : order[j]];
+      CallRay& o = cr[j];
+      o.src[0] = L.src.xpos;
+      o.src[1] = L.src.ypos;
+      o.src[2] = L.src.zpos;
+      o.tau[0] = L.optical_depth_HI;
+      o.tau[1] = L.optical_depth_HeI;
+      o.tau[2] = L.optical_depth_HeII;
+      o.pad = 0;
+      const double* row = L.src.photon_rate;
       SpectrumKey key;
       memcpy(&key.a, &row[0], 8);
       memcpy(&key.b, &row[ng / 2], 8);
