@@ -41,6 +41,7 @@ EXPORTS = [
     "argot_hip_set_iterations", "input_mesh_single", "input_mesh", "input_src_file", "input_src",
     "input_data", "output_mesh_single", "output_mesh", "output_src", "output_data",
     "argot_hip_set_max_sources", "argot_hip_set_chem_queue", "argot_hip_set_wide_index",
+    "argot_hip_set_host_threads",
     "argot_hip_release_gpu",
 ]
 
@@ -150,6 +151,7 @@ def lib():
     L.argot_hip_set_iterations.argtypes = [C.c_int]
     L.argot_hip_set_chem_queue.argtypes = [C.c_int]
     L.argot_hip_set_wide_index.argtypes = [C.c_int]
+    L.argot_hip_set_host_threads.argtypes = [C.c_int]
     vp, cs_ = C.c_void_p, C.c_char_p
     for n in ("input_mesh_single", "input_mesh", "output_mesh_single", "output_mesh"):
         getattr(L, n).argtypes = [vp, rp, cs_]

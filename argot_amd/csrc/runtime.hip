@@ -95,6 +95,7 @@ int g_use_graph = 1;
 int g_iterations = 0;                  // 0: one point-source pass
 int g_wide_index = 0;                  // 1: 64-bit cell-field addresses on any mesh
 int g_chem_queue = 1;                  // cell-queue chemistry: free lanes refill at this count
+int g_host_threads = 16;               // host threads of calc_photoion_rate_cuda (0: serial)
 
 // Everything a captured step graph depends on.
 struct GraphKey {
@@ -676,7 +677,8 @@ struct SpectrumKeyHash {
 // at most the hardware's.
 static int host_threads(int64_t n) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  return (int)std::min<int64_t>(std::min<unsigned>(hw, 16u), std::max<int64_t>(1, n / 4096));
+  const unsigned cap = (unsigned)std::max(1, g_host_threads);
+  return (int)std::min<int64_t>(std::min(hw, cap), std::max<int64_t>(1, n / 4096));
 }
 
 // Runs f(0 .. nt-1), f(0) on the calling thread and the others on threads
@@ -850,7 +852,8 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   CallTarget* ct = reinterpret_cast<CallTarget*>(stage + sizeof(CallRay) * nray);
   int64_t ntarget = 0;
   std::vector<const double*> rows;   // first ray's row of each distinct spectrum
-  if (!number_spectra(ray, contiguous ? nullptr : order.data(), nray, ng, cr, &rows)) {
+  if (g_host_threads == 0 ||
+      !number_spectra(ray, contiguous ? nullptr : order.data(), nray, ng, cr, &rows)) {
     // two different rows with one hash, or a host thread that failed:
     // the records and the numbering again, serially, candidates found by
     // three bins and confirmed on the whole row
@@ -1391,6 +1394,7 @@ void argot_hip_set_iterations(int n) { g_iterations = n > 0 ? n : 0; }
 
 void argot_hip_set_wide_index(int on) { g_wide_index = on ? 1 : 0; }
 void argot_hip_set_chem_queue(int n) { g_chem_queue = n > 0 ? std::min(n, 64) : 0; }
+void argot_hip_set_host_threads(int n) { g_host_threads = n > 0 ? std::min(n, 256) : 0; }
 
 // Copy bandwidth probe: reps device-to-device copies of nbytes with a plain
 // streaming kernel; returns the best GB/s (read + write bytes / time).

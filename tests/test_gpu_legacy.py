@@ -187,6 +187,16 @@ def test_rate_entry_point_equals_fused_step(require_gpu):
     idx3 = np.r_[np.arange(0, mid), np.arange(ends[k], len(t)), np.arange(mid, ends[k])]
     got3 = rate_call(step, light_rays_from_records(rec[idx3], w, spectra), len(idx3))
     assert np.array_equal(got3[fin].view(np.uint32), fused[fin].view(np.uint32))
+    # the rays read and the spectra numbered serially (the path taken when
+    # two different spectra share a hash): the same bits and spectrum count
+    L = lib()
+    L.argot_hip_set_host_threads(0)
+    try:
+        got4 = rate_call(step, light_rays_from_records(rec[idx], w, spectra), len(idx))
+        assert step.stats()["nspec_call"] == ps["nspec_call"]
+    finally:
+        L.argot_hip_set_host_threads(16)
+    assert np.array_equal(got4[fin].view(np.uint32), fused[fin].view(np.uint32))
 
 
 def test_merge_photoion_data(require_gpu):
