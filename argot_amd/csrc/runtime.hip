@@ -903,6 +903,22 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
     }
     ct[ntarget - 1].count++;
   }
+  // Targets in 4x4x4 tiles, so that each wavefront of the kernel holds 64
+  // neighbouring targets, whose rays mostly come from the same nodes and
+  // spectra (as in the fused step); each target's rays keep their order, so
+  // every cell's sum is unchanged.
+  {
+    // tile coordinates, then the place inside the tile (indices below 2^24)
+    auto tile_key = [](const CallTarget& T) {
+      const uint64_t bx = (uint32_t)T.g[0] >> 2, by = (uint32_t)T.g[1] >> 2,
+                     bz = (uint32_t)T.g[2] >> 2;
+      const uint64_t in = (uint64_t)(((T.g[0] & 3) << 4) | ((T.g[1] & 3) << 2) | (T.g[2] & 3));
+      return std::make_pair((bx << 22) | by, (bz << 6) | in);
+    };
+    std::sort(ct, ct + ntarget, [&](const CallTarget& a, const CallTarget& b) {
+      return tile_key(a) < tile_key(b);
+    });
+  }
   const int64_t nspec = (int64_t)rows.size();
   // The tables, as the per-ray kernels of the fused step read a node's
   // spectrum: FP32 values scaled by the row maximum, the maximum in FP64,
