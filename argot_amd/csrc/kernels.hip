@@ -400,6 +400,17 @@ __global__ __launch_bounds__(64) ARGOT_CALL_OCCUPANCY void target_rates_kernel(
   }
 }
 
+// Every third bit of v (bits 0, 3, 6, ...) packed together: one axis of a
+// Morton index.
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x09249249u;
+  v = (v ^ (v >> 2)) & 0x030c30c3u;
+  v = (v ^ (v >> 4)) & 0x0300f00fu;
+  v = (v ^ (v >> 8)) & 0xff0000ffu;
+  v = (v ^ (v >> 16)) & 0x000003ffu;
+  return v;
+}
+
 // Next tile for this wavefront: its own XCD's queue first (blocks are
 // dispatched round-robin over the 8 XCDs), then the others in turn.
 __device__ __forceinline__ int64_t next_tile(StepCounters* ctr, int64_t ntile, int* q, int* tried) {
@@ -485,6 +496,19 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   const int TX = 1 << bx, TY = 1 << by, TZ = 1 << bz;
   const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY, ntz = (g.nz + TZ - 1) / TZ;
   const int64_t ntile = (int64_t)ntx * nty * ntz;
+  // Tiles are handed out in Morton (Z-curve) order over the smallest
+  // power-of-two cube of tiles that holds the mesh, whose places outside
+  // the mesh are skipped: each XCD's queue is then an octant of that cube,
+  // and the wavefronts running at one time on an XCD hold a compact block
+  // of targets whose rays cross the same cells, so that more of their
+  // gathers hit the XCD's L2 (1.08x on the 256^3 / 4096-source step).
+  // A mesh whose cube would be more than 8 times its tile count keeps the
+  // linear order.
+  int mbits = 0;
+  while ((1 << mbits) < max(ntx, max(nty, ntz))) mbits++;
+  const int64_t ncube = (int64_t)1 << (3 * mbits);
+  const bool morton = mbits <= 10 && ncube <= 8 * ntile;
+  const int64_t nqueue_tiles = morton ? ncube : ntile;
   int q = blockIdx.x % kNumQueues, tried = 0;
   // per-lane counts (a lane's rays and crossings over the launch fit in 32
   // bits); wave-uniform visit counts
@@ -505,11 +529,19 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
   }
 
   for (;;) {
-    const int64_t t = next_tile(a.ctr, ntile, &q, &tried);
+    const int64_t t = next_tile(a.ctr, nqueue_tiles, &q, &tried);
     if (t < 0) break;
-    const int tz = (int)(t % ntz);
-    const int ty = (int)((t / ntz) % nty);
-    const int tx = (int)(t / ((int64_t)ntz * nty));
+    int tx, ty, tz;
+    if (morton) {
+      tx = (int)compact3((uint32_t)t >> 2);
+      ty = (int)compact3((uint32_t)t >> 1);
+      tz = (int)compact3((uint32_t)t);
+      if (tx >= ntx || ty >= nty || tz >= ntz) continue;   // outside the mesh
+    } else {
+      tz = (int)(t % ntz);
+      ty = (int)((t / ntz) % nty);
+      tx = (int)(t / ((int64_t)ntz * nty));
+    }
     const int ix = tx * TX + ((lane >> (by + bz)) & (TX - 1));
     const int iy = ty * TY + ((lane >> bz) & (TY - 1));
     const int iz = tz * TZ + (lane & (TZ - 1));
