@@ -288,8 +288,9 @@ struct PackedFields {
 
 // The whole walk of one segment (calc_optical_depth.c:76-227).  fields(idx)
 // gives the per-cell FP32 densities (nHI, nHeI, nHeII) formed as the
-// reference forms them.  The next cell's densities are fetched before the
-// current chord is summed, so one gather is always in flight.  KAHAN selects
+// reference forms them.  The densities of the next two cells are fetched
+// before the current chord is summed (the DDA runs one cell ahead of the
+// sums), so two gathers are in flight.  KAHAN selects
 // compensated sums; HASH records the cell sequence (FNV-1a over the linear
 // cell indices, the end cell last).
 template <bool KAHAN, bool HASH, typename Fields>
@@ -301,29 +302,60 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
   uint64_t h = 0xcbf29ce484222325ull;
-  bool act = dda_active(d);
-  float4 n = fields(act ? d.idx : 0u);
-  while (act) {
+  // The loop is unrolled by two so that the two field slots (A, B) alternate
+  // without register copies (a copy of a load's result would wait for the
+  // load).  Every pass issues both loads, whichever lanes are active, so
+  // that the wait before each sum is for the older load only.  Slot A holds
+  // cell k (always active inside the loop), slot B cell k+1; the chords are
+  // summed in cell order, so the sums are those of the one-slot loop.
+  bool actA = dda_active(d);
+  float4 nA = fields(actA ? d.idx : 0u);
+  float rA = 0.0f, rB = 0.0f;
+  bool actB = false;
+  if (actA) {
     if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
-    nc++;
-    const float rmin = dda_step(&d);
-    const bool nact = dda_active(d);
-    const float4 nn = fields(nact ? d.idx : 0u);
-    if (KAHAN) {
-      kahan_add(t0, c0, rmin * n.x);
-      kahan_add(t1, c1, rmin * n.y);
-      kahan_add(t2, c2, rmin * n.z);
-    } else {
-      t0 = t0 + rmin * n.x;
-      t1 = t1 + rmin * n.y;
-      t2 = t2 + rmin * n.z;
-    }
-    n = nn;
-    act = nact;
+    rA = dda_step(&d);
+    actB = dda_active(d);
   }
+  float4 nB = fields(actB ? d.idx : 0u);
+  auto add = [&](float r, const float4& n) {
+    if (KAHAN) {
+      kahan_add(t0, c0, r * n.x);
+      kahan_add(t1, c1, r * n.y);
+      kahan_add(t2, c2, r * n.z);
+    } else {
+      t0 = t0 + r * n.x;
+      t1 = t1 + r * n.y;
+      t2 = t2 + r * n.z;
+    }
+  };
+  if (actA) do {
+    nc++;
+    add(rA, nA);
+    bool a2 = false;
+    if (actB) {
+      if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+      rB = dda_step(&d);
+      a2 = dda_active(d);
+    }
+    nA = fields(a2 ? d.idx : 0u);
+    bool a3 = false;
+    if (actB) {
+      nc++;
+      add(rB, nB);
+      if (a2) {
+        if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+        rA = dda_step(&d);
+        a3 = dda_active(d);
+      }
+    }
+    nB = fields(a3 ? d.idx : 0u);
+    actA = a2;
+    actB = a3;
+  } while (actA);
   const float depth = dda_depth(g, d, e);
   const uint32_t ie = d.idxe;
-  n = fields(ie);
+  float4 n = fields(ie);
   if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
   nc++;
   if (KAHAN) {
