@@ -141,10 +141,19 @@ ARGOT_HD inline int segment_start(const GridDev& g, Vec3f src, Vec3f tgt, Vec3f*
 // the cell is inside the mesh on that axis while f < lim (lim = n + 1 when
 // moving up, 1 when moving down: the walk never moves back).  The linear
 // cell index moves by the axis stride at each crossing.
+//
+// POW2 (cell sizes that are powers of two): f holds the face position
+// sign * delta * f itself, moved by sign * delta at each crossing, which is
+// exact (an integer below 2^24 times a power of two), so the face distances
+// are bit for bit those of the general form without its multiply.  The cell
+// is inside on an axis while (f < l) != neg, with l = (n + 1/2) delta when
+// moving up and -delta/2 when moving down (f is a multiple of delta, so the
+// comparison never meets equality).
 struct Dda {
-  float fx, fy, fz;      // face counters
+  float fx, fy, fz;      // face counters (POW2: face positions)
   float sdx, sdy, sdz;   // sign * delta
-  float lx, ly, lz;      // inside while f < l
+  float lx, ly, lz;      // inside while f < l (POW2: while (f < l) != neg)
+  bool negx, negy, negz; // POW2: moving down
   int stx, sty, stz;     // signed strides of the linear index
   uint32_t idx;          // linear index of the current cell
   uint32_t idxe;         // linear index of the end cell
@@ -164,6 +173,7 @@ __device__ __forceinline__ uint32_t cell_index(const GridDev& g, int ix, int iy,
   return __umul24(row, (uint32_t)g.nz) + (uint32_t)iz;
 }
 
+template <bool POW2>
 __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   int ixe = (int)((e.x - g.xmin) / g.dx);
   int iye = (int)((e.y - g.ymin) / g.dy);
@@ -200,15 +210,27 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   d->ryo = 1.0 / (double)yo;
   d->rzo = 1.0 / (double)zo;
   const bool px = (double)xo > 0.0, py = (double)yo > 0.0, pz = (double)zo > 0.0;
-  d->fx = px ? (float)ix + 1.0f : -(float)ix;
-  d->fy = py ? (float)iy + 1.0f : -(float)iy;
-  d->fz = pz ? (float)iz + 1.0f : -(float)iz;
   d->sdx = px ? g.dx : -g.dx;
   d->sdy = py ? g.dy : -g.dy;
   d->sdz = pz ? g.dz : -g.dz;
-  d->lx = px ? (float)(g.nx + 1) : 1.0f;
-  d->ly = py ? (float)(g.ny + 1) : 1.0f;
-  d->lz = pz ? (float)(g.nz + 1) : 1.0f;
+  if constexpr (POW2) {
+    d->fx = (px ? (float)ix + 1.0f : (float)ix) * g.dx;
+    d->fy = (py ? (float)iy + 1.0f : (float)iy) * g.dy;
+    d->fz = (pz ? (float)iz + 1.0f : (float)iz) * g.dz;
+    d->lx = (px ? (float)g.nx + 0.5f : -0.5f) * g.dx;
+    d->ly = (py ? (float)g.ny + 0.5f : -0.5f) * g.dy;
+    d->lz = (pz ? (float)g.nz + 0.5f : -0.5f) * g.dz;
+    d->negx = !px;
+    d->negy = !py;
+    d->negz = !pz;
+  } else {
+    d->fx = px ? (float)ix + 1.0f : -(float)ix;
+    d->fy = py ? (float)iy + 1.0f : -(float)iy;
+    d->fz = pz ? (float)iz + 1.0f : -(float)iz;
+    d->lx = px ? (float)(g.nx + 1) : 1.0f;
+    d->ly = py ? (float)(g.ny + 1) : 1.0f;
+    d->lz = pz ? (float)(g.nz + 1) : 1.0f;
+  }
   const int sz = 1, sy = g.nz, sx = g.ny * g.nz;
   d->stx = px ? sx : -sx;
   d->sty = py ? sy : -sy;
@@ -220,23 +242,29 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
 
 // The walk continues while the current cell lies inside the mesh and is not
 // the end cell (the index is meaningful only when the cell is inside).
+template <bool POW2>
 __device__ __forceinline__ bool dda_active(const Dda& d) {
-  return (d.fx < d.lx) & (d.fy < d.ly) & (d.fz < d.lz) & (d.idx != d.idxe);
+  if constexpr (POW2)
+    return ((d.fx < d.lx) != d.negx) & ((d.fy < d.ly) != d.negy) & ((d.fz < d.lz) != d.negz) &
+           (d.idx != d.idxe);
+  else
+    return (d.fx < d.lx) & (d.fy < d.ly) & (d.fz < d.lz) & (d.idx != d.idxe);
 }
 
 // One step of calc_optical_depth.c:163-179; returns the chord length.
+template <bool POW2>
 __device__ __forceinline__ float dda_step(Dda* d) {
-  const float ddx = d->sdx * d->fx - d->xc;
-  const float ddy = d->sdy * d->fy - d->yc;
-  const float ddz = d->sdz * d->fz - d->zc;
+  const float ddx = (POW2 ? d->fx : d->sdx * d->fx) - d->xc;
+  const float ddy = (POW2 ? d->fy : d->sdy * d->fy) - d->yc;
+  const float ddz = (POW2 ? d->fz : d->sdz * d->fz) - d->zc;
   const float rx = div_by(ddx, d->rxo);
   const float ry = div_by(ddy, d->ryo);
   const float rz = div_by(ddz, d->rzo);
   const float rmin = fminf(rx, fminf(ry, rz));
   const bool ax = rmin == rx, ay = rmin == ry, az = rmin == rz;
-  d->fx += ax ? 1.0f : 0.0f;
-  d->fy += ay ? 1.0f : 0.0f;
-  d->fz += az ? 1.0f : 0.0f;
+  d->fx += ax ? (POW2 ? d->sdx : 1.0f) : 0.0f;
+  d->fy += ay ? (POW2 ? d->sdy : 1.0f) : 0.0f;
+  d->fz += az ? (POW2 ? d->sdz : 1.0f) : 0.0f;
   d->idx += (uint32_t)((ax ? d->stx : 0) + (ay ? d->sty : 0) + (az ? d->stz : 0));
   d->xc = d->xc + rmin * d->xo;
   d->yc = d->yc + rmin * d->yo;
@@ -270,8 +298,15 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float term) {
 constexpr int64_t kNarrowCells = (int64_t)1 << 28;
 template <bool NARROW>
 __device__ __forceinline__ float4 cell_fields(const float4* __restrict__ nf, uint32_t idx) {
-  if constexpr (NARROW)
-    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nf) + (idx << 4));
+  if constexpr (NARROW) {
+    // The empty asm keeps the 32-bit byte offset a value of its own, so
+    // that the load takes the scalar base plus a 32-bit lane offset even
+    // where the caller selects the index (left alone, the compiler selects
+    // between two 64-bit addresses instead, two more instructions).
+    uint32_t off = idx << 4;
+    asm("" : "+v"(off));
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nf) + off);
+  }
   else
     return nf[idx];
 }
@@ -292,12 +327,13 @@ struct PackedFields {
 // before the current chord is summed (the DDA runs one cell ahead of the
 // sums), so two gathers are in flight.  KAHAN selects
 // compensated sums; HASH records the cell sequence (FNV-1a over the linear
-// cell indices, the end cell last).
-template <bool KAHAN, bool HASH, typename Fields>
+// cell indices, the end cell last); POW2 selects the face-position form of
+// the DDA for power-of-two cell sizes (same results).
+template <bool KAHAN, bool HASH, bool POW2, typename Fields>
 __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields, Vec3f s, Vec3f e,
                                          WalkOut* out) {
   Dda d;
-  dda_setup(g, s, e, &d);
+  dda_setup<POW2>(g, s, e, &d);
   float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
@@ -308,14 +344,14 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
   // that the wait before each sum is for the older load only.  Slot A holds
   // cell k (always active inside the loop), slot B cell k+1; the chords are
   // summed in cell order, so the sums are those of the one-slot loop.
-  bool actA = dda_active(d);
+  bool actA = dda_active<POW2>(d);
   float4 nA = fields(actA ? d.idx : 0u);
   float rA = 0.0f, rB = 0.0f;
   bool actB = false;
   if (actA) {
     if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
-    rA = dda_step(&d);
-    actB = dda_active(d);
+    rA = dda_step<POW2>(&d);
+    actB = dda_active<POW2>(d);
   }
   float4 nB = fields(actB ? d.idx : 0u);
   auto add = [&](float r, const float4& n) {
@@ -335,8 +371,8 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
     bool a2 = false;
     if (actB) {
       if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
-      rB = dda_step(&d);
-      a2 = dda_active(d);
+      rB = dda_step<POW2>(&d);
+      a2 = dda_active<POW2>(d);
     }
     nA = fields(a2 ? d.idx : 0u);
     bool a3 = false;
@@ -345,8 +381,8 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
       add(rB, nB);
       if (a2) {
         if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
-        rA = dda_step(&d);
-        a3 = dda_active(d);
+        rA = dda_step<POW2>(&d);
+        a3 = dda_active<POW2>(d);
       }
     }
     nB = fields(a3 ? d.idx : 0u);
@@ -375,10 +411,10 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
 }
 
 // The walk over the packed per-cell densities nf (float4 per cell).
-template <bool KAHAN, bool HASH, bool NARROW>
+template <bool KAHAN, bool HASH, bool NARROW, bool POW2>
 __device__ __forceinline__ void walk_segment(const GridDev& g, const float4* __restrict__ nf,
                                              Vec3f s, Vec3f e, WalkOut* out) {
-  walk_segment_with<KAHAN, HASH>(g, PackedFields<NARROW>{nf}, s, e, out);
+  walk_segment_with<KAHAN, HASH, POW2>(g, PackedFields<NARROW>{nf}, s, e, out);
 }
 
 // delta_length, calc_photoion_rate.c:52-90.

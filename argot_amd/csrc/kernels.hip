@@ -9,6 +9,9 @@
 //                       sum for 4x4x4 target tiles, one wavefront per tile
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+#include <cmath>
+
 #include "argot_device.h"
 #include "argot_internal.h"
 #include "kernels.h"
@@ -105,9 +108,10 @@ __global__ void set_rates_kernel(unsigned char* __restrict__ mesh, const float* 
 // wavefront walk segments of similar length; each segment's result does not
 // depend on which lane walks it.
 // Fields: where the per-cell densities come from (PackedFields: the packed
-// float4 array; MeshFields: the mesh records).
+// float4 array; MeshFields: the mesh records).  POW2: the DDA's face-position
+// form for power-of-two cell sizes.
 constexpr int kWalkBlock = 256;
-template <typename Fields>
+template <bool POW2, typename Fields>
 __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
     GridDev g, Fields fields, unsigned char* __restrict__ seg, int64_t nseg, int kahan) {
   __shared__ unsigned bucket_lds[64];
@@ -145,9 +149,9 @@ __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
   const Vec3f e{rec[7], rec[8], rec[9]};
   WalkOut w;
   if (kahan)
-    walk_segment_with<true, false>(g, fields, s, e, &w);
+    walk_segment_with<true, false, POW2>(g, fields, s, e, &w);
   else
-    walk_segment_with<false, false>(g, fields, s, e, &w);
+    walk_segment_with<false, false, POW2>(g, fields, s, e, &w);
   rec[10] = w.tau[0];
   rec[11] = w.tau[1];
   rec[12] = w.tau[2];
@@ -486,7 +490,7 @@ __device__ inline void surr_rates(const RayRate& rr, double q, const float ya[kN
   res[5] = p2 * m[5] * kHnuL;
 }
 
-template <bool KAHAN, bool CAPTURE, bool SURR, bool NARROW>
+template <bool KAHAN, bool CAPTURE, bool SURR, bool NARROW, bool POW2>
 __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArgs a) {
   const GridDev& g = a.grid;
   const TreeDev& tr = a.tree;
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArg
         // a ray the reference refuses (calc_ray_segment.c:212) is counted
         // where it happens, on the path of sources outside the box
         segment_start(g, src, tgt, &st, &a.ctr->noverflow);
-        walk_segment<KAHAN, CAPTURE, NARROW>(g, a.nf, st, tgt, &w);
+        walk_segment<KAHAN, CAPTURE, NARROW, POW2>(g, a.nf, st, tgt, &w);
         float4 f;
         if constexpr (SURR)
           f = make_float4(*mesh_f(a.mesh, cell, kOffDens), *mesh_f(a.mesh, cell, kOffFHI),
@@ -800,24 +804,47 @@ hipError_t launch_add_rates(void* mesh, const float* rate, int64_t ncell, hipStr
   return hipGetLastError();
 }
 
+// Can the walk keep face positions instead of face counters?  Every cell
+// size must be a power of two (then an integer face counter below 2^24
+// times it is exact) and every mesh size below 2^23 (so that n + 1/2 is
+// exact too).
+static bool pow2_cells(const GridDev& g) {
+  const float d[3] = {g.dx, g.dy, g.dz};
+  const int n[3] = {g.nx, g.ny, g.nz};
+  for (int k = 0; k < 3; k++) {
+    int e = 0;
+    if (!(d[k] >= FLT_MIN) || !(d[k] <= 1.0e30f) || std::frexp(d[k], &e) != 0.5f) return false;
+    if (n[k] < 1 || n[k] >= (1 << 23)) return false;
+  }
+  return true;
+}
+
+template <typename Fields>
+static void launch_walk(const GridDev& g, const Fields& f, void* seg, int64_t nseg, int kahan,
+                        bool pow2, hipStream_t s) {
+  if (pow2)
+    walk_segments_kernel<true><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, f, static_cast<unsigned char*>(seg), nseg, kahan);
+  else
+    walk_segments_kernel<false><<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
+        g, f, static_cast<unsigned char*>(seg), nseg, kahan);
+}
+
 hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, int64_t nseg,
                                 int kahan, int wide, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
   if (!wide && (int64_t)g.nx * g.ny * g.nz < kNarrowCells)
-    walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-        g, PackedFields<true>{nf}, static_cast<unsigned char*>(seg), nseg, kahan);
+    launch_walk(g, PackedFields<true>{nf}, seg, nseg, kahan, pow2_cells(g), s);
   else
-    walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-        g, PackedFields<false>{nf}, static_cast<unsigned char*>(seg), nseg, kahan);
+    launch_walk(g, PackedFields<false>{nf}, seg, nseg, kahan, false, s);
   return hipGetLastError();
 }
 
 hipError_t launch_walk_segments_mesh(const GridDev& g, const void* mesh, double helium_fact,
                                      void* seg, int64_t nseg, int kahan, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
-  walk_segments_kernel<<<nblocks(nseg, kWalkBlock), kWalkBlock, 0, s>>>(
-      g, MeshFields{static_cast<const unsigned char*>(mesh), helium_fact},
-      static_cast<unsigned char*>(seg), nseg, kahan);
+  launch_walk(g, MeshFields{static_cast<const unsigned char*>(mesh), helium_fact}, seg, nseg,
+              kahan, pow2_cells(g), s);
   return hipGetLastError();
 }
 
@@ -850,35 +877,42 @@ hipError_t launch_reset_counters(StepCounters* ctr, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <bool SURR, bool NARROW>
+template <bool SURR, bool NARROW, bool POW2>
 static void launch_fused(const FusedArgs& a, int kahan, int capture, int nblock, hipStream_t s) {
   if (capture) {
     if (kahan)
-      fused_step_kernel<true, true, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, true, SURR, NARROW, POW2><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, true, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, true, SURR, NARROW, POW2><<<nblock, 64, 0, s>>>(a);
   } else {
     if (kahan)
-      fused_step_kernel<true, false, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<true, false, SURR, NARROW, POW2><<<nblock, 64, 0, s>>>(a);
     else
-      fused_step_kernel<false, false, SURR, NARROW><<<nblock, 64, 0, s>>>(a);
+      fused_step_kernel<false, false, SURR, NARROW, POW2><<<nblock, 64, 0, s>>>(a);
   }
+}
+
+// Cell-field addressing and face arithmetic: 32-bit offsets on meshes of
+// fewer than 2^28 cells, with face positions where the cell sizes allow;
+// otherwise (and always when wide is set) 64-bit addresses and face counters.
+template <bool SURR>
+static void launch_fused_form(const FusedArgs& a, int kahan, int capture, int wide, int nblock,
+                              hipStream_t s) {
+  const bool narrow = !wide && (int64_t)a.grid.nx * a.grid.ny * a.grid.nz < kNarrowCells;
+  if (narrow && pow2_cells(a.grid))
+    launch_fused<SURR, true, true>(a, kahan, capture, nblock, s);
+  else if (narrow)
+    launch_fused<SURR, true, false>(a, kahan, capture, nblock, s);
+  else
+    launch_fused<SURR, false, false>(a, kahan, capture, nblock, s);
 }
 
 hipError_t launch_fused_step(const FusedArgs& a, int kahan, int capture, int surr, int wide,
                              int nblock, hipStream_t s) {
-  const bool narrow = !wide && (int64_t)a.grid.nx * a.grid.ny * a.grid.nz < kNarrowCells;
-  if (surr) {
-    if (narrow)
-      launch_fused<true, true>(a, kahan, capture, nblock, s);
-    else
-      launch_fused<true, false>(a, kahan, capture, nblock, s);
-  } else {
-    if (narrow)
-      launch_fused<false, true>(a, kahan, capture, nblock, s);
-    else
-      launch_fused<false, false>(a, kahan, capture, nblock, s);
-  }
+  if (surr)
+    launch_fused_form<true>(a, kahan, capture, wide, nblock, s);
+  else
+    launch_fused_form<false>(a, kahan, capture, wide, nblock, s);
   return hipGetLastError();
 }
 

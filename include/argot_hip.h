@@ -370,15 +370,17 @@ void argot_hip_set_iterations(int n);
  * busy), n at most 64; 0: one cell per thread.  Results are identical for
  * every n. */
 void argot_hip_set_chem_queue(int n);
-/* 1: the walks address the per-cell fields with 64-bit offsets on any mesh;
- * 0 (default): with 32-bit byte offsets when the mesh has fewer than 2^28
- * cells, 64-bit ones otherwise.  Results are identical (a test control for
- * the large-mesh path). */
 /* Host threads calc_photoion_rate_cuda uses to read the caller's rays and
  * number their distinct spectra: n >= 1 caps them (default 16; at most one
  * per 4096 rays and per hardware thread); 0 reads and numbers serially.
  * Results are identical. */
 void argot_hip_set_host_threads(int n);
+/* 1: the walks address the per-cell fields with 64-bit offsets and count
+ * cell faces with integer-valued counters on any mesh; 0 (default): 32-bit
+ * byte offsets when the mesh has fewer than 2^28 cells (64-bit otherwise),
+ * and running face positions instead of counters when every cell size is a
+ * power of two.  Results are identical (a test control for the large-mesh
+ * and general-spacing paths). */
 void argot_hip_set_wide_index(int on);
 /* replaces recv_photoion_data, src/recv_photoion_data.cu:14-43:
  * gamma[c] := mesh_dev[c].prev_chem rates (through cuda_mem->rate_dev). */

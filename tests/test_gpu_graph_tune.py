@@ -81,15 +81,18 @@ def test_autotuner_reports_and_keeps_the_fastest(require_gpu, restore_defaults):
     np.testing.assert_array_equal(st.rates(), r0)
 
 
+@pytest.mark.parametrize("n", [24, 32])
 @pytest.mark.parametrize("kahan", [0, 1])
-def test_wide_cell_addresses_give_identical_results(require_gpu, restore_defaults, kahan):
-    """The 64-bit cell-field addressing that meshes of 2^28 cells or more
-    take gives the same rates and crossings as the 32-bit one, in the fused
-    step (through a new graph) and in calc_optical_depth_cuda."""
+def test_wide_cell_addresses_give_identical_results(require_gpu, restore_defaults, kahan, n):
+    """The 64-bit cell-field addressing and face counters that meshes of
+    2^28 cells or more take give the same rates and crossings as the 32-bit
+    offsets, in the fused step (through a new graph) and in
+    calc_optical_depth_cuda; on the 32^3 mesh (cell size 2^-5) the default
+    form also keeps running face positions instead of face counters."""
     import ctypes as C
     from argot_amd import abi
     L = lib()
-    w = small_workload(n=24, nsrc=9, seed=33)
+    w = small_workload(n=n, nsrc=9, seed=33)
     st = R.PointSourceStep(w, tau_mode=kahan)
     s0 = st.step()
     r0 = st.rates()
