@@ -154,9 +154,9 @@ struct Dda {
   float sdx, sdy, sdz;   // sign * delta
   float lx, ly, lz;      // inside while f < l (POW2: while (f < l) != neg)
   bool negx, negy, negz; // POW2: moving down
-  int stx, sty, stz;     // signed strides of the linear index
-  uint32_t idx;          // linear index of the current cell
-  uint32_t idxe;         // linear index of the end cell
+  uint32_t stx, sty, stz;  // strides of the linear index times S, modulo 2^32
+  uint32_t idx;          // linear index of the current cell times S
+  uint32_t idxe;         // linear index of the end cell times S
   float xo, yo, zo;      // direction cosines (with the TINY guards)
   double rxo, ryo, rzo;  // 1/xo etc. in double (exact-division helper)
   float xc, yc, zc;      // position in local coordinates
@@ -173,7 +173,9 @@ __device__ __forceinline__ uint32_t cell_index(const GridDev& g, int ix, int iy,
   return __umul24(row, (uint32_t)g.nz) + (uint32_t)iz;
 }
 
-template <bool POW2>
+// S scales the linear indices the walk carries (16: byte offsets into the
+// packed float4 fields).
+template <bool POW2, uint32_t S>
 __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   int ixe = (int)((e.x - g.xmin) / g.dx);
   int iye = (int)((e.y - g.ymin) / g.dy);
@@ -187,8 +189,8 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
   ix = max(min(ix, g.nx - 1), 0);
   iy = max(min(iy, g.ny - 1), 0);
   iz = max(min(iz, g.nz - 1), 0);
-  d->idxe = cell_index(g, ixe, iye, ize);
-  d->idx = cell_index(g, ix, iy, iz);
+  d->idxe = cell_index(g, ixe, iye, ize) * S;
+  d->idx = cell_index(g, ix, iy, iz) * S;
 
   float dx = e.x - s.x, dy = e.y - s.y, dz = e.z - s.z;
   float dl = (float)__builtin_sqrt((double)(dx * dx + dy * dy + dz * dz) + kWalkTiny);
@@ -231,10 +233,10 @@ __device__ inline void dda_setup(const GridDev& g, Vec3f s, Vec3f e, Dda* d) {
     d->ly = py ? (float)(g.ny + 1) : 1.0f;
     d->lz = pz ? (float)(g.nz + 1) : 1.0f;
   }
-  const int sz = 1, sy = g.nz, sx = g.ny * g.nz;
-  d->stx = px ? sx : -sx;
-  d->sty = py ? sy : -sy;
-  d->stz = pz ? sz : -sz;
+  const uint32_t sz = S, sy = (uint32_t)g.nz * S, sx = (uint32_t)g.ny * (uint32_t)g.nz * S;
+  d->stx = px ? sx : 0u - sx;
+  d->sty = py ? sy : 0u - sy;
+  d->stz = pz ? sz : 0u - sz;
   d->xc = s.x - g.xmin;
   d->yc = s.y - g.ymin;
   d->zc = s.z - g.zmin;
@@ -265,7 +267,7 @@ __device__ __forceinline__ float dda_step(Dda* d) {
   d->fx += ax ? (POW2 ? d->sdx : 1.0f) : 0.0f;
   d->fy += ay ? (POW2 ? d->sdy : 1.0f) : 0.0f;
   d->fz += az ? (POW2 ? d->sdz : 1.0f) : 0.0f;
-  d->idx += (uint32_t)((ax ? d->stx : 0) + (ay ? d->sty : 0) + (az ? d->stz : 0));
+  d->idx += (ax ? d->stx : 0u) + (ay ? d->sty : 0u) + (az ? d->stz : 0u);
   d->xc = d->xc + rmin * d->xo;
   d->yc = d->yc + rmin * d->yo;
   d->zc = d->zc + rmin * d->zo;
@@ -292,32 +294,34 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float term) {
   s = t;
 }
 
-// Fields of cell idx.  NARROW: the byte offset idx*16 fits in 32 bits (the
-// mesh has fewer than 2^28 cells), so the load adds it to the scalar base
-// without a 64-bit address computation.
+// Fields of a cell.  A Fields functor states the scale kScale of the
+// indices it takes (the walk carries the linear cell index times kScale) and
+// whether its loads are range-checked (kChecked: an index outside the mesh
+// reads zeros or another cell's data, never memory outside the array, so
+// the walk need not redirect the loads of lanes that have left the mesh).
 constexpr int64_t kNarrowCells = (int64_t)1 << 28;
-template <bool NARROW>
-__device__ __forceinline__ float4 cell_fields(const float4* __restrict__ nf, uint32_t idx) {
-  if constexpr (NARROW) {
-    // The empty asm keeps the 32-bit byte offset a value of its own, so
-    // that the load takes the scalar base plus a 32-bit lane offset even
-    // where the caller selects the index (left alone, the compiler selects
-    // between two 64-bit addresses instead, two more instructions).
-    uint32_t off = idx << 4;
-    asm("" : "+v"(off));
-    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(nf) + off);
-  }
-  else
-    return nf[idx];
-}
+constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit buffer, gfx9 layout
 
-// Per-cell FP32 densities (nHI, nHeI, nHeII) of cell idx from the packed
-// float4 array.
+// Per-cell FP32 densities (nHI, nHeI, nHeII) from the packed float4 array.
+// NARROW (fewer than 2^28 cells): indices are byte offsets (cell * 16, below
+// 2^32) and the loads are buffer loads against the array's size, which
+// return zeros past its end.
 template <bool NARROW>
 struct PackedFields {
   const float4* __restrict__ nf;
+  uint32_t nbytes;   // NARROW: 16 * cells
+  static constexpr uint32_t kScale = NARROW ? 16u : 1u;
+  static constexpr bool kChecked = NARROW;
   __device__ __forceinline__ float4 operator()(uint32_t idx) const {
-    return cell_fields<NARROW>(nf, idx);
+    if constexpr (NARROW) {
+      typedef uint32_t U3 __attribute__((ext_vector_type(3)));
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<float4*>(nf), (short)0, (int)nbytes, kBufferWord3);
+      const U3 v = __builtin_amdgcn_raw_buffer_load_b96(r, idx, 0, 0);
+      return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), 0.0f);
+    } else {
+      return nf[idx];
+    }
   }
 };
 
@@ -332,8 +336,12 @@ struct PackedFields {
 template <bool KAHAN, bool HASH, bool POW2, typename Fields>
 __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields, Vec3f s, Vec3f e,
                                          WalkOut* out) {
+  constexpr uint32_t S = Fields::kScale;
+  // index of a lane's next load: any index when the loads are range-checked,
+  // else a cell inside the mesh (cell 0 for a lane whose walk has ended)
+  auto at = [&](bool act, uint32_t i) { return (Fields::kChecked || act) ? i : 0u; };
   Dda d;
-  dda_setup<POW2>(g, s, e, &d);
+  dda_setup<POW2, S>(g, s, e, &d);
   float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f;
   float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   int nc = 0;
@@ -345,15 +353,15 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
   // cell k (always active inside the loop), slot B cell k+1; the chords are
   // summed in cell order, so the sums are those of the one-slot loop.
   bool actA = dda_active<POW2>(d);
-  float4 nA = fields(actA ? d.idx : 0u);
+  float4 nA = fields(at(actA, d.idx));
   float rA = 0.0f, rB = 0.0f;
   bool actB = false;
   if (actA) {
-    if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+    if (HASH) h = (h ^ (uint64_t)(d.idx / S)) * 0x100000001b3ull;
     rA = dda_step<POW2>(&d);
     actB = dda_active<POW2>(d);
   }
-  float4 nB = fields(actB ? d.idx : 0u);
+  float4 nB = fields(at(actB, d.idx));
   auto add = [&](float r, const float4& n) {
     if (KAHAN) {
       kahan_add(t0, c0, r * n.x);
@@ -370,29 +378,29 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
     add(rA, nA);
     bool a2 = false;
     if (actB) {
-      if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+      if (HASH) h = (h ^ (uint64_t)(d.idx / S)) * 0x100000001b3ull;
       rB = dda_step<POW2>(&d);
       a2 = dda_active<POW2>(d);
     }
-    nA = fields(a2 ? d.idx : 0u);
+    nA = fields(at(a2, d.idx));
     bool a3 = false;
     if (actB) {
       nc++;
       add(rB, nB);
       if (a2) {
-        if (HASH) h = (h ^ (uint64_t)d.idx) * 0x100000001b3ull;
+        if (HASH) h = (h ^ (uint64_t)(d.idx / S)) * 0x100000001b3ull;
         rA = dda_step<POW2>(&d);
         a3 = dda_active<POW2>(d);
       }
     }
-    nB = fields(a3 ? d.idx : 0u);
+    nB = fields(at(a3, d.idx));
     actA = a2;
     actB = a3;
   } while (actA);
   const float depth = dda_depth(g, d, e);
   const uint32_t ie = d.idxe;
   float4 n = fields(ie);
-  if (HASH) h = (h ^ (uint64_t)ie) * 0x100000001b3ull;
+  if (HASH) h = (h ^ (uint64_t)(ie / S)) * 0x100000001b3ull;
   nc++;
   if (KAHAN) {
     kahan_add(t0, c0, depth * n.x);
@@ -414,7 +422,8 @@ __device__ inline void walk_segment_with(const GridDev& g, const Fields& fields,
 template <bool KAHAN, bool HASH, bool NARROW, bool POW2>
 __device__ __forceinline__ void walk_segment(const GridDev& g, const float4* __restrict__ nf,
                                              Vec3f s, Vec3f e, WalkOut* out) {
-  walk_segment_with<KAHAN, HASH, POW2>(g, PackedFields<NARROW>{nf}, s, e, out);
+  const uint32_t nbytes = NARROW ? (uint32_t)((int64_t)g.nx * g.nyz * 16) : 0u;
+  walk_segment_with<KAHAN, HASH, POW2>(g, PackedFields<NARROW>{nf, nbytes}, s, e, out);
 }
 
 // delta_length, calc_photoion_rate.c:52-90.

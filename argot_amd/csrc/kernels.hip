@@ -58,6 +58,8 @@ __global__ void pack_fields_kernel(const unsigned char* __restrict__ mesh, int64
 struct MeshFields {
   const unsigned char* __restrict__ mesh;
   double helium_fact;
+  static constexpr uint32_t kScale = 1u;
+  static constexpr bool kChecked = false;
   __device__ __forceinline__ float4 operator()(uint32_t idx) const {
     return form_fields(mesh, (int64_t)idx, helium_fact);
   }
@@ -834,9 +836,10 @@ hipError_t launch_walk_segments(const GridDev& g, const float4* nf, void* seg, i
                                 int kahan, int wide, hipStream_t s) {
   if (nseg <= 0) return hipSuccess;
   if (!wide && (int64_t)g.nx * g.ny * g.nz < kNarrowCells)
-    launch_walk(g, PackedFields<true>{nf}, seg, nseg, kahan, pow2_cells(g), s);
+    launch_walk(g, PackedFields<true>{nf, (uint32_t)(g.nx * g.nyz * 16)}, seg, nseg, kahan,
+                pow2_cells(g), s);
   else
-    launch_walk(g, PackedFields<false>{nf}, seg, nseg, kahan, false, s);
+    launch_walk(g, PackedFields<false>{nf, 0u}, seg, nseg, kahan, false, s);
   return hipGetLastError();
 }
 
