@@ -348,14 +348,20 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// ARGOT_WAVES_PER_EU (build option) caps the register budget so that this
-// many wavefronts fit per SIMD.
+// ARGOT_WAVES_PER_EU (build option, 6 by default) caps the register budget
+// of the exact-rate fused kernels so that this many wavefronts fit per SIMD:
+// their walks wait on gathers, and more wavefronts hide more of that wait
+// than the spilled registers cost (on 256^3 / 4096 sources, 140 ms at 6
+// against 143 ms at 5, 148 ms at 4 and 149 ms at 8).  The surrogate variant,
+// whose LDS holds about 4 wavefronts per SIMD, and the per-call rate kernel,
+// which does not walk, keep the budget of 4.
 #if defined(ARGOT_WAVES_PER_EU) && ARGOT_WAVES_PER_EU > 0
-#define ARGOT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(ARGOT_WAVES_PER_EU)))
+#define ARGOT_OCCUPANCY_FOR(SURR) \
+  __attribute__((amdgpu_waves_per_eu((SURR) ? 4 : ARGOT_WAVES_PER_EU)))
 #else
-#define ARGOT_OCCUPANCY
+#define ARGOT_OCCUPANCY_FOR(SURR)
 #endif
-#define ARGOT_CALL_OCCUPANCY ARGOT_OCCUPANCY
+#define ARGOT_CALL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
 
 // calc_photoion_rate_cuda (calc_photoion_rate_cuda.cu:92-216 with the
 // per-target merge of calc_photoion_rate.c:403-425): one lane per target
@@ -508,7 +514,7 @@ __device__ inline void surr_rates(const RayRate& rr, double q, const float ya[kN
 }
 
 template <bool KAHAN, bool CAPTURE, bool SURR, bool NARROW, bool POW2>
-__global__ __launch_bounds__(64) ARGOT_OCCUPANCY void fused_step_kernel(FusedArgs a) {
+__global__ __launch_bounds__(64) ARGOT_OCCUPANCY_FOR(SURR) void fused_step_kernel(FusedArgs a) {
   const GridDev& g = a.grid;
   const TreeDev& tr = a.tree;
   const RateTables& rt = a.rates;

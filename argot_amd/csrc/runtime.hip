@@ -89,7 +89,7 @@ double g_helium_fact = 0.0811258278;   // chemistry.h:25 (Y = 0.245)
 int g_tau_mode = 0;
 int g_rate_mode = 0;
 float g_theta = 0.7f;                  // ARGOT_THETA_CRIT, run_param.h:80
-int g_blocks_per_cu = 16;
+int g_blocks_per_cu = 24;
 int g_tile_bits[3] = {2, 2, 2};        // 4x4x4 targets per wavefront
 int g_use_graph = 1;
 int g_iterations = 0;                  // 0: one point-source pass
@@ -408,7 +408,7 @@ void argot_hip_step_stats(struct argot_step_stats* out) {
 void argot_hip_set_theta(float theta) { g_theta = theta; }
 
 // Persistent grid size of the fused kernel, in wavefronts per CU.
-void argot_hip_set_blocks_per_cu(int n) { g_blocks_per_cu = n > 0 ? n : 16; }
+void argot_hip_set_blocks_per_cu(int n) { g_blocks_per_cu = n > 0 ? n : 24; }
 
 void argot_hip_set_tile(int tx, int ty, int tz) {
   t_err = ARGOT_OK;
@@ -1467,7 +1467,7 @@ void argot_hip_autotune_step(struct fluid_mesh* mesh, struct radiation_src* src,
                              struct cuda_param* this_cuda, struct argot_tune_report* rep) {
   static const int kTiles[][3] = {{4, 4, 4}, {2, 4, 8}, {4, 2, 8}, {1, 8, 8}, {8, 8, 1},
                                   {2, 2, 16}, {8, 4, 2}, {4, 8, 2}};
-  static const int kBlocks[] = {8, 16, 32};
+  static const int kBlocks[] = {16, 24, 32};
   t_err = ARGOT_OK;
   if (!rep) return set_err(ARGOT_ERR_ARG);
   memset(rep, 0, sizeof(*rep));

@@ -664,7 +664,9 @@ void argot_hip_surrogate_set_baseline(const float *coef, int npoly);
 /* Opening angle for step_radiation_tree (default 0.7 = ARGOT_THETA_CRIT). */
 void argot_hip_set_theta(float theta);
 
-/* Persistent grid of the fused kernel, in wavefronts per compute unit. */
+/* Persistent grid of the fused kernel, in wavefronts per compute unit
+ * (default 24: six per SIMD, the exact-rate kernels' register budget;
+ * n <= 0 restores the default). */
 void argot_hip_set_blocks_per_cu(int n);
 
 /* Target tile of one wavefront in the fused kernel: tx*ty*tz = 64 cells,
