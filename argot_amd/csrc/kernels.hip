@@ -289,7 +289,16 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
   // per-ray prefactors first (same operations as after the loop), so that
   // only they, not the ray's FP64 quantities, stay live through the bins
-  const double pre = scale * exp(-tsh);
+  // exp(-tsh) as 2^n 2^f, n = floor(-tsh log2 e) and f in [0, 1): the FP32
+  // exponential of f (relative error about 1e-7, below the loop's own) times
+  // an exact power of two, in FP64 range.  The library exponential's
+  // polynomial would hold a dozen FP64 constants in registers across the
+  // kernel, which the register budget spills.
+  constexpr double kLog2eP = 1.4426950408889634;
+  const double xl = -tsh * kLog2eP;
+  const double fl = fmax(floor(xl), -2000.0);
+  const double pre =
+      scale * ldexp((double)__builtin_amdgcn_exp2f((float)(xl - fl)), (int)fl);
   const double p0 = rr.r[0] * rr.d[0] * pre;
   const double p1 = rr.r[1] * rr.d[1] * pre;
   const double p2 = rr.r[2] * rr.d[2] * pre;
