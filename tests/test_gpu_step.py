@@ -115,10 +115,12 @@ def test_deterministic(require_gpu):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("shape", [(12, 7, 5), (1, 9, 3), (2, 2, 2)])
+@pytest.mark.parametrize("shape", [(12, 7, 5), (1, 9, 3), (2, 2, 2), (16, 8, 4)])
 def test_ragged_mesh_every_ray(require_gpu, shape):
     """Mesh sizes that are not multiples of the target tile and non-cubic
-    cells: every ray, and every cell's rates."""
+    cells: every ray, and every cell's rates.  (16, 8, 4) has cells of three
+    different power-of-two sizes (the walk's face-position form) and tiles
+    handed out in Morton order over a cube 8 times their number."""
     w = ragged_workload(shape, nsrc=4, seed=sum(shape))
     P = oracle_problem(w)
     step = R.PointSourceStep(w)
