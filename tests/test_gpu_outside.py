@@ -35,8 +35,10 @@ def outside_workload(pos, n=16, seed=31):
     return w
 
 
-def test_sources_outside_every_face(require_gpu):
-    w = outside_workload(OUTSIDE)
+@pytest.mark.parametrize("n", [16, 12])
+def test_sources_outside_every_face(require_gpu, n):
+    # 16^3: power-of-two cells (the walk's face positions); 12^3: face counters
+    w = outside_workload(OUTSIDE, n=n)
     P = oracle_problem(w)
     nseg = [len(P.ray_segment(p, 8, 8, 8)[0]) for p in OUTSIDE]
     assert max(nseg) >= 3   # the domain loop runs several steps
