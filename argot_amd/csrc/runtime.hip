@@ -889,17 +889,20 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
       cr[j].spec = sp;
     }
   }
+  // (the global indices come from the local cell index, which was formed
+  // from them, so the caller's records are not read again)
+  const int64_t nyz = (int64_t)g.ny * g.nz;
   for (int64_t j = 0; j < nray; j++) {
     const int64_t i = contiguous ? j : order[j];
     if (j == 0 || tgt[i] != ct[ntarget - 1].cell) {
-      const light_ray& R = ray[i];
+      const int64_t c = tgt[i];
       CallTarget& T = ct[ntarget++];
       T.first = j;
-      T.cell = tgt[i];
+      T.cell = c;
       T.count = 0;
-      T.g[0] = R.ix_target;
-      T.g[1] = R.iy_target;
-      T.g[2] = R.iz_target;
+      T.g[0] = (int32_t)(c / nyz) + gd.ox;
+      T.g[1] = (int32_t)((c / g.nz) % g.ny) + gd.oy;
+      T.g[2] = (int32_t)(c % g.nz) + gd.oz;
     }
     ct[ntarget - 1].count++;
   }
@@ -907,17 +910,30 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   // neighbouring targets, whose rays mostly come from the same nodes and
   // spectra (as in the fused step); each target's rays keep their order, so
   // every cell's sum is unchanged.
+  // A counting sort, least significant key first: by the place inside the
+  // tile (64 values), then stably by the tile (x, then y, then z).  The
+  // targets are distinct cells, so the order is that of (tile, place).
   {
-    // tile coordinates, then the place inside the tile (indices below 2^24)
-    auto tile_key = [](const CallTarget& T) {
-      const uint64_t bx = (uint32_t)T.g[0] >> 2, by = (uint32_t)T.g[1] >> 2,
-                     bz = (uint32_t)T.g[2] >> 2;
-      const uint64_t in = (uint64_t)(((T.g[0] & 3) << 4) | ((T.g[1] & 3) << 2) | (T.g[2] & 3));
-      return std::make_pair((bx << 22) | by, (bz << 6) | in);
+    const int64_t ntx = ((int64_t)g.nx + 3) / 4, nty = ((int64_t)g.ny + 3) / 4,
+                  ntz = ((int64_t)g.nz + 3) / 4;
+    auto local = [&](const CallTarget& T, int d) {
+      return (int64_t)T.g[d] - (d == 0 ? gd.ox : d == 1 ? gd.oy : gd.oz);
     };
-    std::sort(ct, ct + ntarget, [&](const CallTarget& a, const CallTarget& b) {
-      return tile_key(a) < tile_key(b);
-    });
+    auto place = [&](const CallTarget& T) {
+      return (int)(((local(T, 0) & 3) << 4) | ((local(T, 1) & 3) << 2) | (local(T, 2) & 3));
+    };
+    auto tile = [&](const CallTarget& T) {
+      return ((local(T, 0) >> 2) * nty + (local(T, 1) >> 2)) * ntz + (local(T, 2) >> 2);
+    };
+    std::vector<CallTarget> tmp((size_t)ntarget);
+    std::vector<int64_t> pos(64 + 1, 0);
+    for (int64_t q = 0; q < ntarget; q++) pos[place(ct[q]) + 1]++;
+    for (int k = 0; k < 64; k++) pos[k + 1] += pos[k];
+    for (int64_t q = 0; q < ntarget; q++) tmp[pos[place(ct[q])]++] = ct[q];
+    std::vector<int64_t> tpos((size_t)(ntx * nty * ntz) + 1, 0);
+    for (int64_t q = 0; q < ntarget; q++) tpos[tile(tmp[q]) + 1]++;
+    for (size_t k = 0; k + 1 < tpos.size(); k++) tpos[k + 1] += tpos[k];
+    for (int64_t q = 0; q < ntarget; q++) ct[tpos[tile(tmp[q])]++] = tmp[q];
   }
   const int64_t nspec = (int64_t)rows.size();
   // The tables, as the per-ray kernels of the fused step read a node's
