@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <memory>
 #include <mutex>
@@ -725,16 +726,20 @@ static uint64_t row_hash(const double* row, int n) {
   return r;
 }
 
-// Fills the compact ray records cr[j] (ray order[j], or j) and numbers the
-// distinct spectrum rows by first appearance, on up to 16 host threads
-// over contiguous ranges of j: each thread hashes its rows (whole rows),
-// keeps the first ray of each hash and checks every later row of that hash
-// against it byte for byte; the ranges' first rays are then merged in
-// order (checked byte for byte against the row already numbered).  Returns
-// false if two different rows share a hash or a thread failed (the caller
-// then fills the records and numbers the spectra serially).
-static bool number_spectra(const light_ray* ray, const int64_t* order, int64_t nray, int ng,
-                           CallRay* cr, std::vector<const double*>* rows) {
+// Fills the compact ray records cr[j] of the caller's rays j, their target
+// cells tgt[j] (cell(ray) < 0: outside the local mesh, *bad is set), and
+// numbers the distinct spectrum rows by first appearance, on up to 16 host
+// threads over contiguous ranges of j, in one pass over the records: each
+// thread hashes its rows (whole rows), keeps the first ray of each hash
+// and checks every later row of that hash against it byte for byte; the
+// ranges' first rays are then merged in order (checked byte for byte
+// against the row already numbered).  Returns false if two different rows
+// share a hash or a thread failed (the caller then fills the records and
+// numbers the spectra serially).
+extern "C++" template <typename CellF>
+static bool number_spectra(const light_ray* ray, int64_t nray, int ng, const CellF& cell,
+                           int64_t* tgt, std::atomic<bool>* bad, CallRay* cr,
+                           std::vector<const double*>* rows) {
   const int nt = host_threads(nray);
   std::vector<uint64_t> hs(nray);
   struct Range {
@@ -748,7 +753,13 @@ static bool number_spectra(const light_ray* ray, const int64_t* order, int64_t n
     std::unordered_map<uint64_t, int32_t> seen;
     Range& R = rg[t];
     for (int64_t j = j0; j < j1; j++) {
-      const light_ray& L = ray[order ? order[j] : j];
+      const light_ray& L = ray[j];
+      const int64_t c = cell(L);
+      if (c < 0) {
+        bad->store(true);
+        return false;
+      }
+      tgt[j] = c;
       const double* row = L.src.photon_rate;
       const uint64_t h = row_hash(row, ng);
       hs[j] = h;
@@ -810,39 +821,13 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   const int ng = ARGOT_NGRID_NU;
   // Each target's rays, together and in the caller's order
   // (calc_photoion_rate_cuda.cu:108-132 repacks them; the per-target sums
-  // then need no atomics).
-  std::vector<int64_t> tgt(nray);
-  {
-    // target cells on the same host threads as the spectra below
-    const int nt = host_threads(nray);
-    auto cells = [&](int t) -> bool {
-      const int64_t i0 = nray * t / nt, i1 = nray * (t + 1) / nt;
-      for (int64_t i = i0; i < i1; i++) {
-        const int64_t lx = ray[i].ix_target - gd.ox, ly = ray[i].iy_target - gd.oy,
-                      lz = ray[i].iz_target - gd.oz;
-        if (lx < 0 || lx >= g.nx || ly < 0 || ly >= g.ny || lz < 0 || lz >= g.nz) return false;
-        tgt[i] = lz + (int64_t)g.nz * (ly + (int64_t)g.ny * lx);
-      }
-      return true;
-    };
-    if (!run_on_threads(nt, cells)) return set_err(ARGOT_ERR_ARG);
-  }
-  bool contiguous = true;
-  {
-    std::vector<unsigned char> seen(g.ncell, 0);
-    for (int64_t i = 0; i < nray && contiguous; i++) {
-      if (i > 0 && tgt[i] == tgt[i - 1]) continue;
-      if (seen[tgt[i]]) contiguous = false;
-      seen[tgt[i]] = 1;
-    }
-  }
-  std::vector<int64_t> order;
-  if (!contiguous) {
-    order.resize(nray);
-    for (int64_t i = 0; i < nray; i++) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t a, int64_t b) { return tgt[a] < tgt[b]; });
-  }
+  // then need no atomics).  One pass over the caller's records gives the
+  // compact ray records, the target cells and the spectrum numbering.
+  auto cell = [&](const light_ray& L) -> int64_t {
+    const int64_t lx = L.ix_target - gd.ox, ly = L.iy_target - gd.oy, lz = L.iz_target - gd.oz;
+    if (lx < 0 || lx >= g.nx || ly < 0 || ly >= g.ny || lz < 0 || lz >= g.nz) return -1;
+    return lz + (int64_t)g.nz * (ly + (int64_t)g.ny * lx);
+  };
   // Staging in the caller's pinned ray_IO buffer (nray * 2096 bytes): the
   // ray records, then the target records, then the spectrum tables when
   // they fit; the tables go from host memory otherwise.
@@ -851,16 +836,20 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
   CallRay* cr = reinterpret_cast<CallRay*>(stage);
   CallTarget* ct = reinterpret_cast<CallTarget*>(stage + sizeof(CallRay) * nray);
   int64_t ntarget = 0;
+  std::vector<int64_t> tgt(nray);
   std::vector<const double*> rows;   // first ray's row of each distinct spectrum
-  if (g_host_threads == 0 ||
-      !number_spectra(ray, contiguous ? nullptr : order.data(), nray, ng, cr, &rows)) {
+  std::atomic<bool> bad(false);
+  if (g_host_threads == 0 || !number_spectra(ray, nray, ng, cell, tgt.data(), &bad, cr, &rows)) {
+    if (bad.load()) return set_err(ARGOT_ERR_ARG);
     // two different rows with one hash, or a host thread that failed:
     // the records and the numbering again, serially, candidates found by
     // three bins and confirmed on the whole row
     std::unordered_map<SpectrumKey, std::vector<int32_t>, SpectrumKeyHash> index;
     rows.clear();
     for (int64_t j = 0; j < nray; j++) {
-      const light_ray& L = ray[contiguous ? j : order[j]];
+      const light_ray& L = ray[j];
+      tgt[j] = cell(L);
+      if (tgt[j] < 0) return set_err(ARGOT_ERR_ARG);
       CallRay& o = cr[j];
       o.src[0] = L.src.xpos;
       o.src[1] = L.src.ypos;
@@ -888,6 +877,27 @@ static void calc_photoion_rate_cuda_body(struct cuda_mem_space* cuda_mem, struct
       }
       cr[j].spec = sp;
     }
+  }
+  bool contiguous = true;
+  {
+    std::vector<unsigned char> seen(g.ncell, 0);
+    for (int64_t i = 0; i < nray && contiguous; i++) {
+      if (i > 0 && tgt[i] == tgt[i - 1]) continue;
+      if (seen[tgt[i]]) contiguous = false;
+      seen[tgt[i]] = 1;
+    }
+  }
+  // A target whose rays are not together: the records in target order, a
+  // stable sort, so each target's rays keep the caller's order (the
+  // spectrum numbers are labels and stay as they are).
+  std::vector<int64_t> order;
+  if (!contiguous) {
+    order.resize(nray);
+    for (int64_t i = 0; i < nray; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return tgt[a] < tgt[b]; });
+    std::vector<CallRay> tmp(cr, cr + nray);
+    for (int64_t j = 0; j < nray; j++) cr[j] = tmp[order[j]];
   }
   // (the global indices come from the local cell index, which was formed
   // from them, so the caller's records are not read again)
