@@ -521,18 +521,4 @@ __device__ inline void rate_loop_f64(const RateTables& rt, const RayRate& rr, Sp
   out[5] = h2 * kHnuL;
 }
 
-// (1 - exp(-x))/x in FP32: Taylor series sum_n (-x)^n/(n+1)! through x^6
-// below 0.25 (truncation < 2e-9), the direct form above (rounding of
-// 1 - exp(-x) costs at most 3e-7 relative there).
-__device__ __forceinline__ float one_minus_exp_over(float x) {
-  float p = __builtin_fmaf(x, 1.0f / 5040, -1.0f / 720);
-  p = __builtin_fmaf(x, p, 1.0f / 120);
-  p = __builtin_fmaf(x, p, -1.0f / 24);
-  p = __builtin_fmaf(x, p, 1.0f / 6);
-  p = __builtin_fmaf(x, p, -1.0f / 2);
-  p = __builtin_fmaf(x, p, 1.0f);
-  const float q = (1.0f - __expf(-x)) * __builtin_amdgcn_rcpf(x);
-  return x < 0.25f ? p : q;
-}
-
 }  // namespace argot

@@ -165,16 +165,20 @@ __global__ __launch_bounds__(kWalkBlock) void walk_segments_kernel(
 // (wave-uniform bin index); sums run in FP32 over 32 bins, then in FP64.
 constexpr float kLog2e = 1.4426950408889634f;
 
-// sinh(y)/y for 0 <= y <= 2 from yl = -log2(e) y: a degree-4 polynomial in
-// u = y^2 (relative-weighted least-squares fit over [0, 4], error 4.3e-8
-// before rounding, 1.4e-7 after FP32 evaluation), evaluated in yl^2 with
+// sinh(y)/y for 0 <= y <= 2.5 from yl = -log2(e) y: a degree-4 polynomial
+// in u = y^2 (minimax relative error over u in [0, 6.25]: 3.3e-7 before
+// rounding, 4.4e-7 after FP32 evaluation), evaluated in yl^2 with
 // coefficient k scaled by log2(e)^(-2k).
 constexpr double kL2 = 1.4426950408889634 * 1.4426950408889634;
-constexpr float kSh0 = (float)1.0000000433e+00;
-constexpr float kSh1 = (float)(1.6666608170e-01 / kL2);
-constexpr float kSh2 = (float)(8.3345603822e-03 / (kL2 * kL2));
-constexpr float kSh3 = (float)(1.9752823696e-04 / (kL2 * kL2 * kL2));
-constexpr float kSh4 = (float)(3.0116996307e-06 / (kL2 * kL2 * kL2 * kL2));
+constexpr float kSh0 = (float)1.0000003313e+00;
+constexpr float kSh1 = (float)(1.6666356469e-01 / kL2);
+constexpr float kSh2 = (float)(8.3377321051e-03 / (kL2 * kL2));
+constexpr float kSh3 = (float)(1.9630612284e-04 / (kL2 * kL2 * kL2));
+constexpr float kSh4 = (float)(3.1551783390e-06 / (kL2 * kL2 * kL2 * kL2));
+// The polynomial's range: in-cell optical depth 2y <= kThinMax, i.e.
+// yl >= kYlThin.
+constexpr double kThinMax = 5.0;
+constexpr float kYlThin = (float)(-0.5 * kThinMax * 1.4426950408889634);
 __device__ __forceinline__ float sinhc_l(float yl) {
   const float u = yl * yl;
   float p = __builtin_fmaf(u, kSh4, kSh3);
@@ -183,8 +187,8 @@ __device__ __forceinline__ float sinhc_l(float yl) {
   return __builtin_fmaf(u, p, kSh0);
 }
 
-// Bins [k0, k1) of a ray whose in-cell optical depth is at most 4 in every
-// bin: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y with
+// Bins [k0, k1) of a ray whose in-cell optical depth is at most kThinMax in
+// every bin: exp(-tau) (1 - exp(-dtau)) / dtau = exp(-(tau + y)) sinh(y)/y with
 // y = dtau/2, one exponential per bin.  dl holds the ray's in-cell weights
 // times -log2(e)/2, so that their sum with the cross sections is
 // yl = -log2(e) y and 2^(yl - log2(e) tau) = exp(-(tau + y)).  FOLD (bins
@@ -237,28 +241,51 @@ __device__ __forceinline__ void rate_bins_thin(const RateTables& rt, int k0, int
   }
 }
 
-template <int NS, typename SpecF>
-__device__ __forceinline__ void rate_bins(const RateTables& rt, int k0, int k1, float cst,
-                                          const float n[3], const float d[3], SpecF spec,
-                                          double G[3], double H[3]) {
+// Bins [k0, k1) for a wavefront in which some lane's ray has an in-cell
+// optical depth above kThinMax.  A lane flagged thin computes exactly what
+// rate_bins_thin computes (the same operations in the same order), so a
+// cell's rates do not depend on which rays share its wavefront.  The other
+// lanes take, bin by bin, the same thin form where yl >= kYlThin, and
+// otherwise exp(-tau) (1 - exp(-2y)) / (2y) with the optical depth tau
+// summed on its own (no cancellation: exp(-2y) <= exp(-kThinMax)).
+template <int NS, bool FOLD, typename SpecF>
+__device__ __forceinline__ void rate_bins_mixed(const RateTables& rt, int k0, int k1, float cst,
+                                                const float n[3], const float nl[3],
+                                                const float dl[3], bool thin, SpecF spec,
+                                                double G[3], double H[3]) {
+  constexpr float kHalfLog2eN = -0.5f * kLog2e;  // 1/(2y) = kHalfLog2eN / yl
   const int ng = rt.ngrid;
   for (int c = k0; c < k1; c += 32) {
     float g[NS], h[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) g[s] = h[s] = 0.0f;
     const int ce = min(c + 32, k1);
-#pragma unroll 4
+#pragma unroll 2
     for (int k = c; k < ce; k++) {
-      float sg[NS], hs[NS];
-      float tau = cst, dt = 0.0f;
+      float sg[NS], hs[NS], ds[NS];
+      float yl = 0.0f, z, tau = cst;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         sg[s] = ldc(rt.sig + s * ng + k);
         hs[s] = ldc(rt.hsig + s * ng + k);
-        tau = __builtin_fmaf(n[s], ldc(rt.dsig + s * ng + k), tau);
-        dt = __builtin_fmaf(d[s], sg[s], dt);
+        ds[s] = ldc(rt.dsig + s * ng + k);
+        yl = __builtin_fmaf(dl[s], sg[s], yl);
       }
-      const float b = spec(k) * __expf(-tau) * one_minus_exp_over(dt);
+#pragma unroll
+      for (int s = 0; s < NS; s++) tau = __builtin_fmaf(n[s], ds[s], tau);
+      if constexpr (FOLD) {
+        z = yl;
+#pragma unroll
+        for (int s = 0; s < NS; s++) z = __builtin_fmaf(nl[s], ds[s], z);
+      } else {
+        z = __builtin_fmaf(tau, -kLog2e, yl);
+      }
+      const float sk = spec(k);
+      const float bt = sk * __builtin_amdgcn_exp2f(z) * sinhc_l(yl);
+      const float e1 = __builtin_amdgcn_exp2f(tau * -kLog2e);
+      const float om = 1.0f - __builtin_amdgcn_exp2f(yl + yl);
+      const float bk = sk * e1 * om * (__builtin_amdgcn_rcpf(yl) * kHalfLog2eN);
+      const float b = (thin || yl >= kYlThin) ? bt : bk;
 #pragma unroll
       for (int s = 0; s < NS; s++) {
         g[s] = __builtin_fmaf(sg[s], b, g[s]);
@@ -284,7 +311,6 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
                                      SpecF spec, double out[6]) {
   const double tsh = rr.t[0] * rt.cs_last[0] + rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2];
   const float n[3] = {(float)rr.t[0], (float)rr.t[1], (float)rr.t[2]};
-  const float d[3] = {(float)rr.d[0], (float)rr.d[1], (float)rr.d[2]};
   const float c1 = (float)(-(rr.t[1] * rt.cs_last[1] + rr.t[2] * rt.cs_last[2]));
   const float c2 = (float)(-(rr.t[2] * rt.cs_last[2]));
   // per-ray prefactors first (same operations as after the loop), so that
@@ -308,20 +334,21 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   // wavefront (tile shape, grid size); lanes of both kinds in one wavefront
   // run both loops under their masks.
   const double dmax = rr.d[0] * rt.cs_max[0] + rr.d[1] * rt.cs_max[1] + rr.d[2] * rt.cs_max[2];
-  if (dmax <= 4.0) {
-    constexpr double kHalfLog2e = 0.5 * 1.4426950408889634;
-    constexpr double kLog2eD = 1.4426950408889634;
-    const float dl[3] = {(float)(rr.d[0] * -kHalfLog2e), (float)(rr.d[1] * -kHalfLog2e),
-                         (float)(rr.d[2] * -kHalfLog2e)};
-    const float nl[3] = {(float)(rr.t[0] * -kLog2eD), (float)(rr.t[1] * -kLog2eD),
-                         (float)(rr.t[2] * -kLog2eD)};
+  constexpr double kHalfLog2e = 0.5 * 1.4426950408889634;
+  constexpr double kLog2eD = 1.4426950408889634;
+  const float dl[3] = {(float)(rr.d[0] * -kHalfLog2e), (float)(rr.d[1] * -kHalfLog2e),
+                       (float)(rr.d[2] * -kHalfLog2e)};
+  const float nl[3] = {(float)(rr.t[0] * -kLog2eD), (float)(rr.t[1] * -kLog2eD),
+                       (float)(rr.t[2] * -kLog2eD)};
+  const bool thin = dmax <= kThinMax;
+  if (__all(thin)) {
     rate_bins_thin<1, false>(rt, 0, rt.k_he1, c1, n, nl, dl, spec, G, H);
     rate_bins_thin<2, false>(rt, rt.k_he1, rt.k_he2, c2, n, nl, dl, spec, G, H);
     rate_bins_thin<3, true>(rt, rt.k_he2, rt.ngrid, 0.0f, n, nl, dl, spec, G, H);
   } else {
-    rate_bins<1>(rt, 0, rt.k_he1, c1, n, d, spec, G, H);
-    rate_bins<2>(rt, rt.k_he1, rt.k_he2, c2, n, d, spec, G, H);
-    rate_bins<3>(rt, rt.k_he2, rt.ngrid, 0.0f, n, d, spec, G, H);
+    rate_bins_mixed<1, false>(rt, 0, rt.k_he1, c1, n, nl, dl, thin, spec, G, H);
+    rate_bins_mixed<2, false>(rt, rt.k_he1, rt.k_he2, c2, n, nl, dl, thin, spec, G, H);
+    rate_bins_mixed<3, true>(rt, rt.k_he2, rt.ngrid, 0.0f, n, nl, dl, thin, spec, G, H);
   }
   out[0] = p0 * G[0];
   out[1] = p0 * H[0] * kHnuL;

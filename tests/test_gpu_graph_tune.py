@@ -6,9 +6,10 @@ large meshes, and the autotuner reports and keeps the fastest candidate."""
 import numpy as np
 import pytest
 
+from argot_amd import ic
 from argot_amd import radiation as R
 from argot_amd._lib import check, lib
-from helpers import small_workload
+from helpers import oracle_problem, rel_err, small_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +20,7 @@ def restore_defaults():
     L = lib()
     L.argot_hip_set_graph(1)
     L.argot_hip_set_tile(4, 4, 4)
-    L.argot_hip_set_blocks_per_cu(16)
+    L.argot_hip_set_blocks_per_cu(24)
     L.argot_hip_set_wide_index(0)
 
 
@@ -57,6 +58,55 @@ def test_tile_shapes_give_identical_rates(require_gpu, restore_defaults, tile):
     s1 = st.step()
     np.testing.assert_array_equal(st.rates(), r0)
     assert (s0["nray"], s0["ncross"]) == (s1["nray"], s1["ncross"])
+
+
+def thin_mesh_with_thick_cells(n=16, nsrc=6, seed=34):
+    """Mostly neutral-poor gas (in-cell optical depth below 1 in every bin)
+    with six dense cells (above 25): most 4x4x4 tiles hold thin cells only,
+    the six that hold a dense cell mix both kinds, and other tile shapes
+    group the same cells differently."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    dens = np.ones(shape, np.float32)
+    for c in [(1, 2, 3), (6, 9, 4), (10, 5, 13), (13, 14, 1), (4, 12, 10), (9, 0, 8)]:
+        dens[c] = 60.0
+    fHI = np.full(shape, 0.05, np.float32)
+    fHeI = np.full(shape, 0.05, np.float32)
+    fHII = (1 - fHI).astype(np.float32)
+    fHeII = (1 - fHeI).astype(np.float32)
+    pos = ((rng.integers(0, n, (nsrc, 3)) + 0.5) / n).astype(np.float32)
+    w = ic.Workload(f"thick{n}", n, dens, fHI, fHII, fHeI, fHeII, np.zeros(shape, np.float32),
+                    pos, ic.power_law_rates(rng, nsrc), np.full(nsrc, 1.0e5, np.float32))
+    # in-cell optical depth at the HI and HeI edges (6.3e-18 and 7.4e-18 cm^2)
+    # over a chord of one to sqrt(3) cell sizes
+    cell = w.lunit / n
+    nh = dens.astype(np.float64) * w.denstonh
+    depth = nh * cell * (fHI * 6.3e-18 + w.helium_fact * fHeI * 7.4e-18)
+    assert (depth[dens == 1.0] * np.sqrt(3.0)).max() < 1.0
+    assert depth[dens == 60.0].min() > 25.0
+    return w
+
+
+def test_thin_rays_beside_thick_ones(require_gpu, restore_defaults):
+    """A thin ray's rates are the same bits whether its wavefront runs the
+    thin loop alone or the combined loop for a thick neighbour, and both
+    kinds of cells match the oracle."""
+    L = lib()
+    w = thin_mesh_with_thick_cells()
+    st = R.PointSourceStep(w)
+    L.argot_hip_set_tile(4, 4, 4)
+    st.step()
+    r0 = st.rates()
+    for tile in [(1, 8, 8), (2, 2, 16), (8, 8, 1)]:
+        L.argot_hip_set_tile(*tile)
+        check("argot_hip_set_tile")
+        st.step()
+        np.testing.assert_array_equal(st.rates(), r0, err_msg=f"tile {tile}")
+    r32, _, _ = oracle_problem(w).step(f64=True)
+    fin = np.isfinite(r32)
+    assert np.array_equal(fin, np.isfinite(r0))
+    err = rel_err(r0[fin], r32[fin], 1e-30)
+    assert err.max() < 1e-5, f"max rel err {err.max():.3e}"
 
 
 def test_bad_tile_is_rejected(require_gpu, restore_defaults):
