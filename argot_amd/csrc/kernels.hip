@@ -329,10 +329,11 @@ __device__ inline void rate_loop_f32(const RateTables& rt, const RayRate& rr, do
   const double p1 = rr.r[1] * rr.d[1] * pre;
   const double p2 = rr.r[2] * rr.d[2] * pre;
   double G[3] = {0, 0, 0}, H[3] = {0, 0, 0};
-  // largest in-cell optical depth of this ray over the bins.  The choice is
-  // the lane's own, so a cell's rates do not depend on which rays share its
-  // wavefront (tile shape, grid size); lanes of both kinds in one wavefront
-  // run both loops under their masks.
+  // largest in-cell optical depth of this ray over the bins.  A wavefront
+  // whose rays are all thin runs the thin loop; one with a thicker ray runs
+  // the combined loop, in which a thin lane does the thin loop's operations
+  // exactly, so a cell's rates do not depend on which rays share its
+  // wavefront (tile shape, grid size).
   const double dmax = rr.d[0] * rt.cs_max[0] + rr.d[1] * rt.cs_max[1] + rr.d[2] * rt.cs_max[2];
   constexpr double kHalfLog2e = 0.5 * 1.4426950408889634;
   constexpr double kLog2eD = 1.4426950408889634;
